@@ -1,0 +1,235 @@
+// Interpolation matrix G: construction (host -> device layouts) and the
+// apply_direct_operator / apply_adjoint_operator entry points.
+#include <algorithm>
+#include <numeric>
+
+#include "gmat.hpp"
+
+namespace fhs {
+
+namespace {
+
+template <class T>
+T* upload(const std::vector<T>& h) {
+  T* d = nullptr;
+  FHS_HIP(hipMalloc(&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) FHS_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+// CSC (column pointers, rows, values) of a COO list; rows ascending per column.
+void build_csc(int64_t ncols, const std::vector<int>& rows, const std::vector<int>& cols,
+               const std::vector<float2>& vals, std::vector<int>& colptr,
+               std::vector<int>& crow, std::vector<float2>& cval) {
+  const size_t nnz = rows.size();
+  colptr.assign(ncols + 1, 0);
+  for (size_t e = 0; e < nnz; ++e) colptr[cols[e] + 1]++;
+  for (int64_t k = 0; k < ncols; ++k) colptr[k + 1] += colptr[k];
+  std::vector<int> pos(colptr.begin(), colptr.end() - 1);
+  crow.resize(nnz);
+  cval.resize(nnz);
+  // entries arrive row-major (rows ascending) so each column stays sorted
+  for (size_t e = 0; e < nnz; ++e) {
+    const int p = pos[cols[e]]++;
+    crow[p] = rows[e];
+    cval[p] = vals[e];
+  }
+}
+
+__global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
+                               const float* __restrict__ lam, float2* __restrict__ y) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.M) return;
+  float2 acc = g_row_dot(g, m, Fx);
+  if (lam) acc = cscale(acc, lam[m]);
+  y[m] = acc;
+}
+
+__global__ void k_gmat_adjoint(GDev g, const float2* __restrict__ y,
+                               const float* __restrict__ lam, float2* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.ncols) return;
+  float2 acc;
+  if (lam)
+    acc = g_col_dot(g, k, [&](int r) { return cscale(y[r], lam[r]); });
+  else
+    acc = g_col_dot(g, k, [&](int r) { return y[r]; });
+  out[k] = acc;
+}
+
+}  // namespace
+
+}  // namespace fhs
+
+using namespace fhs;
+
+extern "C" {
+
+int fhs_gmat_create_csr_host(fhs_ctx* ctx, int64_t M, int64_t ncols, const int64_t* rowptr,
+                             const int64_t* colidx, const float* vals, fhs_gmat** out) {
+  if (!ctx || !out) return FHS_ERR_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    FHS_CHECK(M > 0 && ncols > 0 && M < (1ll << 31) && ncols < (1ll << 31),
+              "gmat_create_csr: M and ncols must be in [1, 2^31)");
+    FHS_CHECK(rowptr[0] == 0, "gmat_create_csr: rowptr[0] must be 0");
+    const int64_t nnz = rowptr[M];
+    FHS_CHECK(nnz >= 0 && nnz < (1ll << 31), "gmat_create_csr: nnz out of range");
+    int K = 0;
+    for (int64_t m = 0; m < M; ++m) {
+      FHS_CHECK(rowptr[m + 1] >= rowptr[m], "gmat_create_csr: rowptr not monotone");
+      K = std::max<int>(K, (int)(rowptr[m + 1] - rowptr[m]));
+    }
+    K = std::max(K, 1);
+    std::vector<int> ell_c((size_t)K * M, 0);
+    std::vector<float2> ell_v((size_t)K * M, make_float2(0.f, 0.f));
+    std::vector<int> rows(nnz), cols(nnz);
+    std::vector<float2> v(nnz);
+    for (int64_t m = 0; m < M; ++m) {
+      for (int64_t e = rowptr[m]; e < rowptr[m + 1]; ++e) {
+        const int64_t c = colidx[e];
+        FHS_CHECK(c >= 0 && c < ncols, "gmat_create_csr: column index out of range");
+        const int j = (int)(e - rowptr[m]);
+        const float2 val = make_float2(vals[2 * e], vals[2 * e + 1]);
+        ell_c[(size_t)j * M + m] = (int)c;
+        ell_v[(size_t)j * M + m] = val;
+        rows[e] = (int)m;
+        cols[e] = (int)c;
+        v[e] = val;
+      }
+    }
+    std::vector<int> colptr, crow;
+    std::vector<float2> cval;
+    build_csc(ncols, rows, cols, v, colptr, crow, cval);
+    FHS_HIP(hipSetDevice(ctx->device));
+    fhs_gmat* g = new fhs_gmat();
+    g->device = ctx->device;
+    g->M = M;
+    g->ncols = ncols;
+    g->nnz = nnz;
+    g->mem[0] = upload(ell_c);
+    g->mem[1] = upload(ell_v);
+    g->mem[2] = upload(colptr);
+    g->mem[3] = upload(crow);
+    g->mem[4] = upload(cval);
+    GDev& d = g->d;
+    d.layout = G_ELL;
+    d.M = (int)M;
+    d.K = K;
+    d.ncols = (int)ncols;
+    d.cols = (const int*)g->mem[0];
+    d.vals = (const float2*)g->mem[1];
+    d.colptr = (const int*)g->mem[2];
+    d.csc_rows = (const int*)g->mem[3];
+    d.csc_vals = (const float2*)g->mem[4];
+    *out = g;
+  });
+}
+
+int fhs_gmat_create_window_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, int Ky, int Kx,
+                                const int32_t* anchor, const float* coef, fhs_gmat** out) {
+  if (!ctx || !out) return FHS_ERR_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    FHS_CHECK(M > 0 && M < (1ll << 31), "gmat_create_window: M out of range");
+    FHS_CHECK(Ky >= 1 && Kx >= 1 && Ky <= Kty && Kx <= Ktx, "gmat_create_window: bad window");
+    const int64_t ncols = (int64_t)Kty * Ktx;
+    FHS_CHECK(ncols < (1ll << 31), "gmat_create_window: grid too large");
+    const int K = Ky * Kx;
+    const int64_t nnz = M * K;
+    FHS_CHECK(nnz < (1ll << 31), "gmat_create_window: nnz out of range");
+    std::vector<float2> wv((size_t)K * M);
+    std::vector<int> rows(nnz), cols(nnz);
+    std::vector<float2> v(nnz);
+    std::vector<int> anc(anchor, anchor + M);
+    for (int64_t m = 0; m < M; ++m) {
+      const int a = anchor[m];
+      FHS_CHECK(a >= 0 && a < ncols, "gmat_create_window: anchor out of range");
+      const int ky0 = a % Kty, kx0 = a / Kty;
+      for (int jx = 0; jx < Kx; ++jx) {
+        for (int jy = 0; jy < Ky; ++jy) {
+          const int j = jx * Ky + jy;
+          const float2 val = make_float2(coef[2 * (m * K + j)], coef[2 * (m * K + j) + 1]);
+          wv[(size_t)j * M + m] = val;
+          const int ky = (ky0 + jy) % Kty, kx = (kx0 + jx) % Ktx;
+          const int64_t e = m * K + j;
+          rows[e] = (int)m;
+          cols[e] = ky + Kty * kx;
+          v[e] = val;
+        }
+      }
+    }
+    std::vector<int> colptr, crow;
+    std::vector<float2> cval;
+    build_csc(ncols, rows, cols, v, colptr, crow, cval);
+    FHS_HIP(hipSetDevice(ctx->device));
+    fhs_gmat* g = new fhs_gmat();
+    g->device = ctx->device;
+    g->M = M;
+    g->ncols = ncols;
+    g->nnz = nnz;
+    g->mem[0] = upload(anc);
+    g->mem[1] = upload(wv);
+    g->mem[2] = upload(colptr);
+    g->mem[3] = upload(crow);
+    g->mem[4] = upload(cval);
+    GDev& d = g->d;
+    d.layout = G_WINDOW;
+    d.M = (int)M;
+    d.K = K;
+    d.Kty = Kty;
+    d.Ktx = Ktx;
+    d.Ky = Ky;
+    d.Kx = Kx;
+    d.ncols = (int)ncols;
+    d.anchor = (const int*)g->mem[0];
+    d.vals = (const float2*)g->mem[1];
+    d.colptr = (const int*)g->mem[2];
+    d.csc_rows = (const int*)g->mem[3];
+    d.csc_vals = (const float2*)g->mem[4];
+    *out = g;
+  });
+}
+
+int fhs_gmat_destroy(fhs_gmat* g) {
+  if (!g) return FHS_OK;
+  (void)hipSetDevice(g->device);
+  for (void* p : g->mem)
+    if (p) (void)hipFree(p);
+  delete g;
+  return FHS_OK;
+}
+
+int fhs_gmat_info(const fhs_gmat* g, int64_t* M, int64_t* ncols, int64_t* nnz) {
+  if (!g) return FHS_ERR_ARG;
+  if (M) *M = g->M;
+  if (ncols) *ncols = g->ncols;
+  if (nnz) *nnz = g->nnz;
+  return FHS_OK;
+}
+
+int fhs_apply_direct_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* Fx,
+                              const float* Lambda, void* y) {
+  if (!ctx) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(G && Fx && y, "apply_direct_operator: null argument");
+    Scope sc(ctx, "gmat_forward");
+    k_gmat_forward<<<cdiv(G->M, 256), 256, 0, ctx->stream>>>(
+        G->d, (const float2*)Fx, Lambda, (float2*)y);
+    check_launch();
+  });
+}
+
+int fhs_apply_adjoint_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* y,
+                               const float* Lambda, void* Fx) {
+  if (!ctx) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(G && Fx && y, "apply_adjoint_operator: null argument");
+    Scope sc(ctx, "gmat_adjoint");
+    k_gmat_adjoint<<<cdiv(G->ncols, 256), 256, 0, ctx->stream>>>(
+        G->d, (const float2*)y, Lambda, (float2*)Fx);
+    check_launch();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Device-resident interpolation matrix G of one channel.
+//
+// Forward (degridding, y = G Fx) reads G transposed: value j of row m sits at
+// vals[j*M + m], so the 64 lanes of a wavefront (64 consecutive rows) load
+// 64 consecutive complex values per tap (one 512-byte coalesced load).
+// Two row formats share that value layout:
+//   ELL    : explicit column per tap (cols[j*M + m]), any sparsity pattern;
+//   WINDOW : one anchor per row; tap j = jx*Ky + jy reads grid point
+//            ((ky0 + jy) mod Kty, (kx0 + jx) mod Ktx) - the dense min-max
+//            window, no column indices (8 B per nonzero instead of 12).
+// Adjoint (gridding, Fx = G' y) is computed per grid point from a CSC copy
+// (colptr / rows / conj-ready values): each output is owned by one thread,
+// no atomics, deterministic summation order.
+#pragma once
+
+#include "common.hpp"
+
+namespace fhs {
+
+enum GLayout { G_ELL = 0, G_WINDOW = 1 };
+
+struct GDev {
+  int layout;
+  int M, K;
+  int Kty, Ktx, Ky, Kx;
+  int ncols;
+  const int* cols;      // ELL: [K][M]
+  const int* anchor;    // WINDOW: [M]
+  const float2* vals;   // [K][M]
+  const int* colptr;    // [ncols + 1]
+  const int* csc_rows;  // [nnz]
+  const float2* csc_vals;
+};
+
+__device__ __forceinline__ float2 g_row_dot(const GDev& g, int m, const float2* __restrict__ Fx) {
+  float2 acc = make_float2(0.f, 0.f);
+  if (g.layout == G_ELL) {
+    for (int j = 0; j < g.K; ++j) {
+      const int c = g.cols[(size_t)j * g.M + m];
+      const float2 v = g.vals[(size_t)j * g.M + m];
+      const float2 f = Fx[c];
+      acc.x += v.x * f.x - v.y * f.y;
+      acc.y += v.x * f.y + v.y * f.x;
+    }
+  } else {
+    const int a = g.anchor[m];
+    const int ky0 = a % g.Kty, kx0 = a / g.Kty;
+    int j = 0;
+    for (int jx = 0; jx < g.Kx; ++jx) {
+      int kx = kx0 + jx;
+      if (kx >= g.Ktx) kx -= g.Ktx;
+      const float2* col = Fx + (size_t)kx * g.Kty;
+      for (int jy = 0; jy < g.Ky; ++jy, ++j) {
+        int ky = ky0 + jy;
+        if (ky >= g.Kty) ky -= g.Kty;
+        const float2 v = g.vals[(size_t)j * g.M + m];
+        const float2 f = col[ky];
+        acc.x += v.x * f.x - v.y * f.y;
+        acc.y += v.x * f.y + v.y * f.x;
+      }
+    }
+  }
+  return acc;
+}
+
+// sum over rows r of column k: conj(G[r,k]) * w(r)
+template <class W>
+__device__ __forceinline__ float2 g_col_dot(const GDev& g, int k, W&& w) {
+  float2 acc = make_float2(0.f, 0.f);
+  const int e0 = g.colptr[k], e1 = g.colptr[k + 1];
+  for (int e = e0; e < e1; ++e) {
+    const float2 v = g.csc_vals[e];
+    const float2 y = w(g.csc_rows[e]);
+    acc.x += v.x * y.x + v.y * y.y;
+    acc.y += v.x * y.y - v.y * y.x;
+  }
+  return acc;
+}
+
+}  // namespace fhs
+
+struct fhs_gmat {
+  int device = 0;
+  fhs::GDev d{};
+  int64_t M = 0, ncols = 0, nnz = 0;
+  void* mem[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};

@@ -1,0 +1,14 @@
+"""fhs: MI355X-native Faceted HyperSARA primal-dual iteration.
+
+Host-side mirror of the reference's hot-path functions (MATLAB names and
+argument meaning) over the C ABI of libfhs.so (include/fhs.h).
+"""
+from ._lib import FhsError, lib  # noqa: F401
+from .core import Context, GMatrix, NufftPlan, default_context  # noqa: F401
+from .operators import (  # noqa: F401
+    apply_adjoint_operator,
+    apply_direct_operator,
+    apply_scaled_fourier_transform,
+    fhs_update_primal,
+    proj_l2ball,
+)

@@ -1,0 +1,86 @@
+"""ctypes binding of libfhs.so (the C ABI declared in include/fhs.h).
+
+The library is loaded from this directory (built in-tree by
+``faceted-hypersara_amd/build.py``).  There is no fallback: if the library is
+missing or was built without a symbol, loading fails with an error naming it.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libfhs.so')
+
+c_void_p = C.c_void_p
+c_int = C.c_int
+c_i64 = C.c_int64
+c_double = C.c_double
+c_float_p = C.POINTER(C.c_float)
+c_double_p = C.POINTER(C.c_double)
+c_i64_p = C.POINTER(C.c_int64)
+c_i32_p = C.POINTER(C.c_int32)
+c_int_p = C.POINTER(C.c_int)
+c_long_p = C.POINTER(C.c_long)
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+PROTOTYPES = {
+    'fhs_ctx_create': [c_int, C.POINTER(c_void_p)],
+    'fhs_ctx_destroy': [c_void_p],
+    'fhs_last_error': [c_void_p],
+    'fhs_ctx_set_stream': [c_void_p, c_void_p],
+    'fhs_ctx_synchronize': [c_void_p],
+    'fhs_ctx_set_timing': [c_void_p, c_int],
+    'fhs_ctx_timing_report': [c_void_p, C.c_char_p, c_int, c_double_p, c_long_p, c_int, c_int_p],
+    'fhs_ctx_timing_reset': [c_void_p],
+    'fhs_gmat_create_csr_host': [c_void_p, c_i64, c_i64, c_i64_p, c_i64_p, c_float_p, C.POINTER(c_void_p)],
+    'fhs_gmat_create_window_host': [c_void_p, c_i64, c_int, c_int, c_int, c_int, c_i32_p, c_float_p,
+                                    C.POINTER(c_void_p)],
+    'fhs_gmat_destroy': [c_void_p],
+    'fhs_gmat_info': [c_void_p, c_i64_p, c_i64_p, c_i64_p],
+    'fhs_apply_direct_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    'fhs_apply_adjoint_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    'fhs_nufft_create_host': [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p,
+                              C.POINTER(c_void_p)],
+    'fhs_nufft_destroy': [c_void_p],
+    'fhs_apply_scaled_fourier_transform': [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
+    'fhs_apply_scaled_fourier_adjoint': [c_void_p, c_void_p, c_void_p, c_int, c_float_p, c_void_p],
+    'fhs_proj_l2ball': [c_void_p, c_void_p, c_double, c_void_p, c_i64, c_void_p],
+    'fhs_update_primal': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
+                          c_double_p, c_double_p],
+}
+_RESTYPE = {'fhs_last_error': C.c_char_p}
+
+
+class FhsError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises FhsError if it is absent or incomplete)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FhsError(f'libfhs.so not found at {LIB_PATH}: build it with '
+                       f'python faceted-hypersara_amd/build.py')
+    L = C.CDLL(LIB_PATH)
+    for name, args in PROTOTYPES.items():
+        try:
+            f = getattr(L, name)
+        except AttributeError as e:
+            raise FhsError(f'libfhs.so lacks symbol {name}; rebuild it') from e
+        f.argtypes = args
+        f.restype = _RESTYPE.get(name, c_int)
+    _lib = L
+    return L
+
+
+def check(status, ctx_ptr=None, what=''):
+    if status != 0:
+        msg = ''
+        if ctx_ptr is not None:
+            raw = lib().fhs_last_error(ctx_ptr)
+            msg = raw.decode() if raw else ''
+        raise FhsError(f'{what} failed (status {status}): {msg}')

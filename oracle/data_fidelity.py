@@ -1,0 +1,133 @@
+"""Oracle: data-fidelity operators and dual update, float64.
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Cells ``{L}{nblocks}`` are nested Python lists; ``G[i][j]`` is a scipy
+sparse matrix (M_b x nnz(W)), ``W[i][j]`` an index array into the
+oversampled grid (MATLAB logical mask -> ``np.flatnonzero``), ``A``/``At``
+callables for one channel (``oracle.nufft.NufftOracle``).
+"""
+import numpy as np
+
+from .prox import sc, solver_proj_elipse_fb
+
+
+def apply_direct_operator(Fx, G, Lambda=None):
+    """``src/apply_direct_operator.m:20``: ``Lambda .* (G * Fx)``."""
+    y = G @ Fx
+    return y if Lambda is None else Lambda * y
+
+
+def apply_adjoint_operator(y, G, Lambda=None):
+    """``src/apply_adjoint_operator.m:21``: ``G' * (Lambda .* y)``."""
+    yy = y if Lambda is None else Lambda * y
+    return G.conj().T @ yy
+
+
+def apply_scaled_fourier_transform(x, A, No):
+    """``src/apply_scaled_fourier_transform.m:21-24``: column l = A(x(:,:,l))."""
+    L = x.shape[2]
+    Fx = np.zeros((No, L), dtype=np.complex128)
+    for l in range(L):
+        Fx[:, l] = A(x[:, :, l])
+    return Fx
+
+
+def _is_tril(G):
+    coo = G.tocoo()
+    return bool(np.all(coo.row >= coo.col))
+
+
+def update_dual_data_fidelity(v2, y, x, Fx_old, proj, A, At, G, W, pU, epsilon,
+                              elipse_proj_max_iter, elipse_proj_min_iter,
+                              elipse_proj_eps, sigma22,
+                              flag_dimensionality_reduction=False, Lambda=None):
+    """``src/update_dual_data_fidelity.m:78-147`` (both branches).
+
+    ``A``/``At``: per-channel lists of callables (the reference passes one
+    handle; channels of a synthetic cube have their own operators).
+    Returns (v2, Ftx, Fx_old, proj, norm_res, global_norm_res, norm_epsilon)
+    where ``global_norm_res`` and ``norm_epsilon`` are sums of squares.
+    """
+    n_channels = x.shape[2]
+    Ftx = np.zeros(x.shape)
+    v2 = [list(b) for b in v2]
+    proj = [list(b) for b in proj]
+    Fx_old = Fx_old.copy()
+    norm_res = []
+    global_norm_res = 0.0
+    norm_epsilon = 0.0
+    for i in range(n_channels):
+        Fx = A[i](x[:, :, i])
+        g2 = np.zeros(Fx.shape, dtype=np.complex128)
+        nr = []
+        for j in range(len(G[i])):
+            Gij, Wij = G[i][j], W[i][j]
+            if flag_dimensionality_reduction:
+                lam = Lambda[i][j]
+                fxp = 2 * Fx[Wij] - Fx_old[Wij, i]
+                if _is_tril(Gij):
+                    hf = Gij @ fxp + Gij.conj().T @ fxp
+                else:
+                    hf = Gij @ fxp
+                v2[i][j] = sc(v2[i][j] + lam * hf - y[i][j], epsilon[i][j])
+                if _is_tril(Gij):
+                    u2 = lam * v2[i][j]
+                    g2[Wij] += Gij.conj().T @ u2 + Gij @ u2
+                else:
+                    g2[Wij] += Gij.conj().T @ (lam * v2[i][j])
+                fs = Fx[Wij]
+                if _is_tril(Gij):
+                    res = lam * (Gij @ fs + Gij.conj().T @ fs) - y[i][j]
+                else:
+                    res = lam * (Gij @ fs) - y[i][j]
+                nr.append(float(np.sqrt(np.sum(np.abs(res) ** 2))))
+            else:
+                r2 = Gij @ (2 * Fx[Wij] - Fx_old[Wij, i])
+                proj[i][j], _ = solver_proj_elipse_fb(
+                    v2[i][j] / pU[i][j], r2, y[i][j], pU[i][j], epsilon[i][j],
+                    proj[i][j], elipse_proj_max_iter, elipse_proj_min_iter,
+                    elipse_proj_eps)
+                v2[i][j] = v2[i][j] + pU[i][j] * (r2 - proj[i][j])
+                g2[Wij] += Gij.conj().T @ v2[i][j]
+                nr.append(float(np.linalg.norm(Gij @ Fx[Wij] - y[i][j])))
+            global_norm_res += nr[-1] ** 2
+            norm_epsilon += epsilon[i][j] ** 2
+        norm_res.append(nr)
+        Fx_old[:, i] = Fx
+        Ftx[:, :, i] = sigma22[i] * np.real(At[i](g2))
+    return v2, Ftx, Fx_old, proj, norm_res, global_norm_res, norm_epsilon
+
+
+def compute_residual_images(x, y, A, At, G, W, flag_visibility_gridding=False, Lambda=None):
+    """``src/compute_residual_images.m:39-103`` (non-DR branch and DR branch).
+
+    ``real(At(sum_j G'(y - G Fx)))`` normalised by the PSF peak (the same
+    chain applied to a Dirac at pixel (Ny/2, Nx/2), 0-based).
+    """
+    Ny, Nx, L = x.shape
+    out = np.zeros(x.shape)
+    dirac = np.zeros((Ny, Nx))
+    dirac[Ny // 2, Nx // 2] = 1.0
+    for i in range(L):
+        Fx = A[i](x[:, :, i])
+        Fd = A[i](dirac)
+        r = np.zeros(Fx.shape, dtype=np.complex128)
+        rd = np.zeros(Fx.shape, dtype=np.complex128)
+        for j in range(len(G[i])):
+            Gij, Wij = G[i][j], W[i][j]
+            if flag_visibility_gridding:
+                lam = Lambda[i][j]
+                tril = _is_tril(Gij)
+                def H(f):
+                    return Gij @ f + Gij.conj().T @ f if tril else Gij @ f
+                def Ht(f):
+                    return Gij.conj().T @ f + Gij @ f if tril else Gij.conj().T @ f
+                res_f = lam * (y[i][j] - lam * H(Fx[Wij]))
+                r[Wij] += Ht(res_f)
+                rd[Wij] += Ht(lam ** 2 * H(Fd[Wij]))
+            else:
+                r[Wij] += Gij.conj().T @ (y[i][j] - Gij @ Fx[Wij])
+                rd[Wij] += Gij.conj().T @ (Gij @ Fd[Wij])
+        out[:, :, i] = np.real(At[i](r)) / np.max(np.real(At[i](rd)))
+    return out

@@ -1,0 +1,110 @@
+"""GPU parity of the data-term operators against the FP64 oracle.
+
+Tolerance: relative error in the 2-norm below 1e-5 (FP32 operators vs FP64
+oracle on the same float32-rounded inputs)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle.nufft import NufftOracle
+from oracle.prox import proj_l2ball as proj_ref, update_primal as primal_ref
+
+TOL = 1e-5
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def uv(rng, M, sigma=np.pi / 4):
+    u = rng.normal(0, sigma, 3 * M); v = rng.normal(0, sigma, 3 * M)
+    keep = (np.abs(u) < np.pi) & (np.abs(v) < np.pi)
+    return u[keep][:M], v[keep][:M]
+
+
+def crand(rng, n):
+    return (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64)
+
+
+def to_dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.cuda() if dtype is None else t.to(dtype).cuda()
+
+
+@pytest.mark.parametrize('Ny,Nx,M', [(32, 32, 500), (64, 48, 3000), (256, 256, 40000)])
+@pytest.mark.parametrize('layout', ['csr', 'window'])
+def test_direct_and_adjoint(ctx, Ny, Nx, M, layout):
+    import fhs
+    rng = np.random.default_rng(Ny * 7 + M)
+    u, v = uv(rng, M)
+    op = NufftOracle(u, v, Ny, Nx)
+    if layout == 'csr':
+        G = fhs.GMatrix.from_scipy(ctx, op.G)
+    else:
+        G = fhs.GMatrix.from_window(ctx, op.anchor, op.coef, op.Kty, op.Ktx, op.Ky, op.Kx)
+    Gr = op.G.astype(np.complex64).astype(np.complex128)   # the matrix the GPU holds
+    Fx = crand(rng, op.No)
+    y = fhs.apply_direct_operator(to_dev(Fx), G).cpu().numpy()
+    assert rel(y, Gr @ Fx.astype(np.complex128)) < TOL
+    lam = rng.uniform(0.5, 2.0, M).astype(np.float32)
+    y2 = fhs.apply_direct_operator(to_dev(Fx), G, to_dev(lam)).cpu().numpy()
+    assert rel(y2, lam * (Gr @ Fx.astype(np.complex128))) < TOL
+    yy = crand(rng, M)
+    g = fhs.apply_adjoint_operator(to_dev(yy), G).cpu().numpy()
+    assert rel(g, Gr.conj().T @ yy.astype(np.complex128)) < TOL
+    g2 = fhs.apply_adjoint_operator(to_dev(yy), G, to_dev(lam)).cpu().numpy()
+    assert rel(g2, Gr.conj().T @ (lam * yy.astype(np.complex128))) < TOL
+
+
+@pytest.mark.parametrize('Ny,Nx', [(8, 8), (16, 32), (64, 64), (100, 60), (256, 256), (512, 512), (1024, 512)])
+def test_scaled_fourier(ctx, Ny, Nx):
+    import fhs
+    rng = np.random.default_rng(Ny + Nx)
+    op = NufftOracle(np.zeros(1), np.zeros(1), Ny, Nx)
+    Kty, Ktx = 1 << int(np.ceil(np.log2(2 * Ny))), 1 << int(np.ceil(np.log2(2 * Nx)))
+    assert (Kty, Ktx) == (op.Kty, op.Ktx) or Ny == 100
+    if Ny == 100:
+        pytest.skip('grid must be a power of two')
+    plan = fhs.NufftPlan(ctx, Ny, Nx, op.Kty, op.Ktx, op.sy, op.sx)
+    L = 3
+    x = rng.random((L, Nx, Ny)).astype(np.float32)
+    Fx = plan.A(to_dev(x)).cpu().numpy()
+    for l in range(L):
+        ref = op.A(x[l].T.astype(np.float64))      # oracle takes (Ny, Nx)
+        assert rel(Fx[l], ref) < TOL
+    g = np.stack([crand(rng, op.No) for _ in range(L)])
+    scale = [1.0, 0.5, 2.0]
+    out = plan.At_real(to_dev(g), scale).cpu().numpy()
+    for l in range(L):
+        ref = scale[l] * np.real(op.At(g[l].astype(np.complex128)))
+        assert rel(out[l].T, ref) < TOL
+
+
+@pytest.mark.parametrize('n,inside', [(1000, False), (1 << 20, False), (5000, True), (1, False)])
+def test_proj_l2ball(ctx, n, inside):
+    import fhs
+    rng = np.random.default_rng(n)
+    x = crand(rng, n); y = crand(rng, n)
+    d = np.linalg.norm((x - y).astype(np.complex128))
+    eps = 2 * d if inside else 0.3 * d
+    p = fhs.proj_l2ball(to_dev(x), eps, to_dev(y)).cpu().numpy()
+    ref = proj_ref(x.astype(np.complex128), eps, y.astype(np.complex128))
+    assert rel(p, ref) < TOL
+    if inside:
+        assert rel(p, x) < 1e-7
+
+
+@pytest.mark.parametrize('n,parts', [(4096, 1), (1 << 20, 3), (12345, 2)])
+def test_update_primal(ctx, n, parts):
+    import fhs
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n).astype(np.float32)
+    gs = [rng.normal(size=n).astype(np.float32) for _ in range(parts)]
+    xd = to_dev(x)
+    xd, xhat, rel_x, norm_x = fhs.fhs_update_primal(xd, *[to_dev(g) for g in gs])
+    xr, xhr, rr, nr = primal_ref(x.astype(np.float64), sum(g.astype(np.float64) for g in gs))
+    assert rel(xd.cpu().numpy(), xr) < TOL
+    assert rel(xhat.cpu().numpy(), xhr) < TOL
+    assert abs(rel_x - rr) / rr < 1e-6 and abs(norm_x - nr) / nr < 1e-6
