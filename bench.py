@@ -1,0 +1,201 @@
+"""Benchmark: HyperSARA PDFB iterations on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): HyperSARA,
+L = 16 channels at nu_l = nu_0 (1 + 0.05 l), 512 x 512 image, base coverage
+of 200k Gaussian uv points scaled by nu_l/nu_max with 10-30 % random
+per-channel flagging, 4 data blocks per channel, 7x7 min-max NUFFT with 2x
+oversampling, db1-db8 + Dirac at 4 levels, nuclear + weighted l21 priors,
+uniform-weight preconditioning with the ellipsoid projection, input SNR 40 dB,
+FP32 operators with FP64 norms.  Synthetic seeded data.
+
+A step is one PDFB iteration (primal update, low-rank, sparsity and data
+dual steps over all 16 channels, as in src/hs/hyperSARA.m:565-617) with its
+stopping-rule scalars read back.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, 'faceted-hypersara_amd')):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = 'time-to-converged cube (s) at ref SNR−0.1 dB, chosen vs plain form; iters/s'
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--size', type=int, default=512)
+    ap.add_argument('--channels', type=int, default=16)
+    ap.add_argument('--vis', type=int, default=200000)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-solve', action='store_true', help='skip the time-to-SNR solve')
+    ap.add_argument('--solve-seconds', type=float, default=60.0)
+    ap.add_argument('--seed', type=int, default=0)
+    return ap.parse_args()
+
+
+def algorithmic_bytes(meas, sara_s):
+    """Per-launch algorithmic HBM bytes of the kernels (formulas of the
+    design notes: complex = 8 B, real FP32 = 4 B)."""
+    L, No = meas.L, meas.nufft.No
+    M = [g.M for g in meas.G]
+    nnz = [g.nnz for g in meas.G]
+    N = meas.Ny * meas.Nx
+    g_bytes = sum(8 * k + 4 * m + 8 * No + 8 * m for k, m in zip(nnz, M))
+    return {
+        'data_adjoint': g_bytes,
+        'data_forward': g_bytes,
+        'fft_fwd_y': 4 * N * L + 8 * (No // 2) * L,          # read image, write half-grid
+        'fft_fwd_x': 8 * (No // 2) * L + 8 * No * L,
+        'fft_adj_x': 8 * No * L + 8 * (No // 2) * L,
+        'fft_adj_y': 8 * (No // 2) * L + 4 * N * L,
+        'wavelet_analysis': 4 * L * N + 8 * L * sara_s,
+        'wavelet_synthesis': 4 * L * sara_s + 4 * L * N,
+        'l21_threshold': 8 * L * sara_s + 4 * L * N + 4 * sara_s,
+        'primal': 4 * L * N * 6,
+        'fb_dual': 44 * sum(M),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1',
+                          'n_gpus': args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import fhs
+    from fhs import solvers, synth
+
+    ctx = fhs.default_context(0)
+    t_setup = time.perf_counter()
+    prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
+    meas = solvers.Measurement(ctx, prob, precond=True)
+    son = meas.op_norm(False)
+    nu2 = meas.op_norm(True)
+    setup_s = time.perf_counter() - t_setup
+    param = {'squared_operator_norm': son, 'nu2': nu2, 'reweighting_max_iter': 3}
+
+    # ---- throughput: steps through the hipGraph engine --------------------
+    holder = {}
+
+    def grab(t, eng, info):
+        holder['eng'] = eng
+        return t < args.warmup
+
+    solvers.hyper_sara(meas, param, callback=grab, max_total_iter=args.warmup + 1)
+    eng = holder['eng']
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.iterate(1)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    iters_per_s = args.steps / elapsed
+
+    # ---- live per-kernel timing (HIP events on the engine stream) ---------
+    ctx.timing_reset()
+    ctx.set_timing(True)
+    n_timed = 10
+    eng.iterate(n_timed)
+    ctx.set_timing(False)
+    rep = ctx.timing_report()
+    ctx.timing_reset()
+    alg = algorithmic_bytes(meas, eng.sara.s)
+    per_iter_ms = {k: v[0] / n_timed for k, v in rep.items()}
+    dominant = max((k for k in rep if k in alg), key=lambda k: rep[k][0])
+    avg_ms = rep[dominant][0] / rep[dominant][1]
+    achieved = alg[dominant] / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {'ms_per_launch': rep[k][0] / rep[k][1], 'launches_per_iter': rep[k][1] / n_timed,
+                   'GBps': (alg[k] / (rep[k][0] / rep[k][1] * 1e-3) / 1e9) if k in alg else None}
+               for k in rep}
+
+    # ---- time to reference SNR (plain FP32 form, zero start) --------------
+    solve = None
+    if not args.no_solve:
+        track = {'snr': [], 't': []}
+        tstart = [None]
+
+        def cb(t, eng2, info):
+            if tstart[0] is None:
+                tstart[0] = time.perf_counter()
+            if t % 10 == 0:
+                x = solvers._uncube(eng2.x)
+                track['snr'].append(solvers.snr_db(prob.X0, x))
+                track['t'].append(time.perf_counter() - tstart[0])
+            return True
+
+        xs, sinfo = solvers.hyper_sara(meas, param, X0=prob.X0, callback=cb, time_limit=args.solve_seconds)
+        final_snr = solvers.snr_db(prob.X0, xs)
+        target = final_snr - 0.1
+        t_hit = next((tt for s, tt in zip(track['snr'], track['t']) if s >= target), sinfo['wall'])
+        solve = {'iterations': sinfo['iterations'], 'converged': sinfo['converged'],
+                 'wall_s': round(sinfo['wall'], 3), 'final_snr_db': round(final_snr, 3),
+                 'time_to_final_snr_minus_0.1dB_s': round(t_hit, 3), 'reweights': sinfo['reweights']}
+
+    # ---- CPU baseline: FP64 oracle, bounded sample -------------------------
+    cpu = None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle.solvers import OracleProblem, hypersara as hs_ref
+        blocks = []
+        for l in range(prob.L):
+            b = np.concatenate([[0], np.cumsum(prob.blocks[l])])
+            blocks.append([(int(b[i]), int(b[i + 1])) for i in range(len(prob.blocks[l]))])
+        allp = meas.pU.cpu().numpy().astype(np.float64)
+        off = np.concatenate([[0], np.cumsum([len(u) for u in prob.u])])
+        pU = [allp[off[l]:off[l + 1]] for l in range(prob.L)]
+        with threadpool_limits(limits=1):
+            op = OracleProblem(prob.Ny, prob.Nx, prob.u, prob.v, prob.y, pU, blocks, prob.epsilon,
+                               g_scale=[1.0 / s for s in prob.sigma_noise])
+            sig, sig_bar = solvers.noise_levels_hs(prob.Ny, prob.Nx, prob.L, np.ones(prob.L), son)
+            pr = dict(solvers.DEFAULT_PARAM)
+            pr.update({'nu2': nu2, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar,
+                       'alpha': 1.0})
+            n_cpu = 1
+            tc = time.perf_counter()
+            hs_ref(op, pr, max_iter_total=n_cpu)
+            tcpu = time.perf_counter() - tc
+        cpu = {'value': round(n_cpu / tcpu, 5), 'unit': 'iters/s', 'cores': 1, 'kind': 'port',
+               'sample': f'{n_cpu} HyperSARA iteration(s) of the same 512x512x16 problem by the FP64 '
+                         f'numpy oracle, single thread ({tcpu:.1f} s)'}
+
+    out = {
+        'metric': METRIC,
+        'value': round(iters_per_s, 3),
+        'unit': 'iters/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(1e3 * elapsed / args.steps, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 (f64 norms, f64 Gram/eig)',
+        'data': 'synthetic',
+        'config': {'workload': f'HyperSARA {args.size}x{args.size}x{args.channels}, '
+                               f'{args.vis} base vis/channel, 10-30% flags, 4 blocks/channel',
+                   'Mtot': int(meas.D.Mtot), 'nnz_G_total': int(sum(g.nnz for g in meas.G)),
+                   'setup_s': round(setup_s, 2)},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1),
+                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
+                     'traffic': None, 'avg_launch_ms': round(avg_ms, 5),
+                     'algorithmic_bytes_per_launch': int(alg[dominant])},
+        'kernels': kernels,
+        'solve_plain_fp32': solve,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -32,15 +32,17 @@ struct GDev {
   const float2* csc_vals;
 };
 
-__device__ __forceinline__ float2 g_row_dot(const GDev& g, int m, const float2* __restrict__ Fx) {
+// sum over taps j of row m: G[m, c_j] * f(c_j)
+template <class F>
+__device__ __forceinline__ float2 g_row_apply(const GDev& g, int m, F&& f) {
   float2 acc = make_float2(0.f, 0.f);
   if (g.layout == G_ELL) {
     for (int j = 0; j < g.K; ++j) {
       const int c = g.cols[(size_t)j * g.M + m];
       const float2 v = g.vals[(size_t)j * g.M + m];
-      const float2 f = Fx[c];
-      acc.x += v.x * f.x - v.y * f.y;
-      acc.y += v.x * f.y + v.y * f.x;
+      const float2 x = f(c);
+      acc.x += v.x * x.x - v.y * x.y;
+      acc.y += v.x * x.y + v.y * x.x;
     }
   } else {
     const int a = g.anchor[m];
@@ -49,18 +51,22 @@ __device__ __forceinline__ float2 g_row_dot(const GDev& g, int m, const float2* 
     for (int jx = 0; jx < g.Kx; ++jx) {
       int kx = kx0 + jx;
       if (kx >= g.Ktx) kx -= g.Ktx;
-      const float2* col = Fx + (size_t)kx * g.Kty;
+      const int colbase = kx * g.Kty;
       for (int jy = 0; jy < g.Ky; ++jy, ++j) {
         int ky = ky0 + jy;
         if (ky >= g.Kty) ky -= g.Kty;
         const float2 v = g.vals[(size_t)j * g.M + m];
-        const float2 f = col[ky];
-        acc.x += v.x * f.x - v.y * f.y;
-        acc.y += v.x * f.y + v.y * f.x;
+        const float2 x = f(colbase + ky);
+        acc.x += v.x * x.x - v.y * x.y;
+        acc.y += v.x * x.y + v.y * x.x;
       }
     }
   }
   return acc;
+}
+
+__device__ __forceinline__ float2 g_row_dot(const GDev& g, int m, const float2* __restrict__ Fx) {
+  return g_row_apply(g, m, [&](int c) { return Fx[c]; });
 }
 
 // sum over rows r of column k: conj(G[r,k]) * w(r)

@@ -18,15 +18,8 @@
 #include <cmath>
 
 #include "common.hpp"
-
-struct fhs_nufft {
-  int device = 0;
-  int Ny = 0, Nx = 0, Kty = 0, Ktx = 0;
-  float* sy = nullptr;
-  float* sx = nullptr;
-  float2* twy = nullptr;
-  float2* twx = nullptr;
-};
+#include "kernels.hpp"
+#include "nufft.hpp"
 
 namespace fhs {
 namespace {

@@ -1,0 +1,226 @@
+// PDFB iteration engine shared by SARA, HyperSARA and Faceted HyperSARA:
+// one call runs whole iterations on the GPU
+// (src/fhs/facetHyperSARA.m:701-783, src/hs/hyperSARA.m:565-617,
+//  src/sara/sara.m:521-633):
+//   1. xsol <- max(xsol - g0 - g1 - g2, 0), xhat = 2 xsol - xprev  (+ FP64 norms)
+//   2. low-rank dual step (global or faceted)           -> g0 = sigma00 w v0
+//   3. SARA l21 dual step                                -> g1 = sigma11 Psi v1
+//   4. data-fidelity step (A, G, ellipsoid, G', At)     -> g2 = sigma22 Re At g2
+// Per iteration four FP64 scalars come back: sum (dx)^2, sum x^2,
+// sum norm_res^2, sum eps^2 (the reference's rel_x, norm_x,
+// norm_residual_check^2, norm_epsilon_check^2).
+//
+// The engine owns a stream created with default flags and priority, so it
+// orders with work on the null stream.  After one eager iteration has sized
+// every workspace buffer, the kernel sequence of an iteration is captured in
+// a hipGraph and replayed; parameters that change (thresholds after a
+// reweighting, epsilon) live in device memory, so replays need no re-capture.
+#include <algorithm>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace fhs {
+void lowrank_dual_step(fhs_ctx* ctx, fhs_lowrank* R, float* v0, const float* xhat, float gscale, float* g0);
+void lowrank_set_thresh(fhs_ctx* ctx, fhs_lowrank* R, const double* h_thresh);
+void sara_dual_step(fhs_ctx* ctx, const fhs_sara* S, float* v1, const float* xhat, int L, const float* w,
+                    double beta, float sigma1, float* g1);
+int64_t sara_ncoefs(const fhs_sara* S);
+void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const float* x, float2* gx_old,
+               float2* proj, const float* pU, int max_it, int min_it, double tol, int dr, const float* lam,
+               float* Ftx, double* d_out2);
+void data_set_sigma22(fhs_ctx* ctx, fhs_data* D, const float* h_sigma22);
+void data_set_epsilon(fhs_ctx* ctx, fhs_data* D, const double* h_eps);
+void data_get_norm_res(fhs_ctx* ctx, fhs_data* D, double* h_out);
+int data_nblocks(const fhs_data* D);
+int data_channels(const fhs_data* D);
+int lowrank_facets(const fhs_lowrank* R);
+int lowrank_channels(const fhs_lowrank* R);
+}  // namespace fhs
+
+struct fhs_pdfb {
+  fhs_ctx* ctx = nullptr;
+  fhs_ctx own{};            // private context state bound to the engine stream
+  fhs_data* D = nullptr;
+  fhs_sara* S = nullptr;
+  fhs_lowrank* R = nullptr;
+  fhs_pdfb_buffers b{};
+  int L = 0, Ny = 0, Nx = 0;
+  int64_t N = 0;
+  double beta1 = 0.0;
+  float sigma00 = 0.f, sigma11 = 0.f;
+  int fb_max = 20, fb_min = 1, dr = 0;
+  double fb_eps = 1e-8;
+  double* d_scal = nullptr;   // [4]
+  double* h_scal = nullptr;   // pinned [4]
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+  bool params_set = false;
+  long iters = 0;
+};
+
+namespace fhs {
+namespace {
+
+void run_iteration(fhs_pdfb* P) {
+  fhs_ctx* c = &P->own;
+  const auto& b = P->b;
+  primal_update(c, b.x, b.g1, b.g2, b.g0, b.xhat, P->N * P->L, P->d_scal);
+  if (P->R) lowrank_dual_step(c, P->R, b.v0, b.xhat, P->sigma00, b.g0);
+  sara_dual_step(c, P->S, b.v1, b.xhat, P->L, b.weights1, P->beta1, P->sigma11, b.g1);
+  data_step(c, P->D, (float2*)b.v2, (const float2*)b.y, b.x, (float2*)b.Gx_old, (float2*)b.proj, b.pU,
+            P->fb_max, P->fb_min, P->fb_eps, P->dr, b.Lambda, b.g2, P->d_scal + 2);
+}
+
+void drop_graph(fhs_pdfb* P) {
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  P->exec = nullptr;
+  P->graph = nullptr;
+}
+
+}  // namespace
+}  // namespace fhs
+
+using namespace fhs;
+
+extern "C" {
+
+int fhs_pdfb_create(fhs_ctx* ctx, fhs_data* D, fhs_sara* S, fhs_lowrank* R,
+                    const fhs_pdfb_buffers* bufs, fhs_pdfb** out) {
+  if (!ctx || !out) return FHS_ERR_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    FHS_CHECK(D && S && bufs, "pdfb_create: data and sparsity prior are required");
+    const fhs_pdfb_buffers& b = *bufs;
+    FHS_CHECK(b.x && b.xhat && b.g1 && b.g2 && b.v1 && b.v2 && b.Gx_old && b.y,
+              "pdfb_create: missing buffer");
+    FHS_CHECK(!R || (b.g0 && b.v0), "pdfb_create: the low-rank prior needs g0 and v0");
+    fhs_pdfb* P = new fhs_pdfb();
+    P->ctx = ctx;
+    P->own.device = ctx->device;
+    FHS_HIP(hipSetDevice(ctx->device));
+    FHS_HIP(hipStreamCreate(&P->own.stream));
+    P->D = D;
+    P->S = S;
+    P->R = R;
+    P->b = b;
+    P->L = data_channels(D);
+    P->Ny = b.Ny;
+    P->Nx = b.Nx;
+    P->N = (int64_t)b.Ny * b.Nx;
+    if (R) FHS_CHECK(lowrank_channels(R) == P->L, "pdfb_create: low-rank prior has a different L");
+    FHS_HIP(hipMalloc(&P->d_scal, 4 * sizeof(double)));
+    FHS_HIP(hipHostMalloc(&P->h_scal, 4 * sizeof(double), hipHostMallocDefault));
+    *out = P;
+  });
+}
+
+int fhs_pdfb_destroy(fhs_pdfb* P) {
+  if (!P) return FHS_OK;
+  (void)hipSetDevice(P->own.device);
+  (void)hipStreamSynchronize(P->own.stream);
+  drop_graph(P);
+  (void)hipFree(P->d_scal);
+  (void)hipHostFree(P->h_scal);
+  P->own.ws.release();
+  for (auto e : P->own.event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(P->own.stream);
+  delete P;
+  return FHS_OK;
+}
+
+int fhs_pdfb_set_params(fhs_pdfb* P, const double* thresh0, double beta1, float sigma00, float sigma11,
+                        const float* sigma22, const double* epsilon, int elipse_proj_max_iter,
+                        int elipse_proj_min_iter, double elipse_proj_eps, int flag_dimensionality_reduction) {
+  if (!P) return FHS_ERR_ARG;
+  fhs_ctx* c = &P->own;
+  int st = guard(P->ctx, [&] {
+    FHS_CHECK(sigma22 && epsilon, "pdfb_set_params: sigma22 and epsilon are required");
+    FHS_CHECK(!P->R || thresh0, "pdfb_set_params: thresh0 required with the low-rank prior");
+    FHS_CHECK(elipse_proj_max_iter >= 1 && elipse_proj_min_iter >= 0, "pdfb_set_params: bad ellipsoid bounds");
+    if (P->R) {
+      const int Q = lowrank_facets(P->R);
+      std::vector<double> th((size_t)Q * 16, 0.0);
+      for (int q = 0; q < Q; ++q)
+        for (int i = 0; i < P->L; ++i) th[(size_t)q * 16 + i] = thresh0[(size_t)q * P->L + i];
+      lowrank_set_thresh(c, P->R, th.data());
+    }
+    data_set_sigma22(c, P->D, sigma22);
+    data_set_epsilon(c, P->D, epsilon);
+    FHS_HIP(hipStreamSynchronize(c->stream));
+    const bool structural = P->params_set &&
+                            (P->beta1 != beta1 || P->sigma00 != sigma00 || P->sigma11 != sigma11 ||
+                             P->fb_max != elipse_proj_max_iter || P->fb_min != elipse_proj_min_iter ||
+                             P->fb_eps != elipse_proj_eps || P->dr != flag_dimensionality_reduction);
+    if (structural) drop_graph(P);   // scalars baked into kernel arguments
+    P->beta1 = beta1;
+    P->sigma00 = sigma00;
+    P->sigma11 = sigma11;
+    P->fb_max = elipse_proj_max_iter;
+    P->fb_min = elipse_proj_min_iter;
+    P->fb_eps = elipse_proj_eps;
+    P->dr = flag_dimensionality_reduction != 0;
+    P->params_set = true;
+  });
+  return st;
+}
+
+int fhs_pdfb_set_graph(fhs_pdfb* P, int enable) {
+  if (!P) return FHS_ERR_ARG;
+  P->use_graph = enable != 0;
+  if (!enable) drop_graph(P);
+  return FHS_OK;
+}
+
+int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
+  if (!P) return FHS_ERR_ARG;
+  fhs_ctx* c = &P->own;
+  int st = guard(P->ctx, [&] {
+    FHS_CHECK(P->params_set, "pdfb_iterate: call fhs_pdfb_set_params first");
+    FHS_CHECK(n_iter >= 1, "pdfb_iterate: n_iter >= 1");
+    c->timing = P->ctx->timing;
+    for (int it = 0; it < n_iter; ++it) {
+      const bool graph_now = P->use_graph && !c->timing && P->iters >= 1;
+      if (graph_now && !P->exec) {
+        FHS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        run_iteration(P);
+        FHS_HIP(hipStreamEndCapture(c->stream, &P->graph));
+        FHS_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+      }
+      if (graph_now)
+        FHS_HIP(hipGraphLaunch(P->exec, c->stream));
+      else
+        run_iteration(P);
+      FHS_HIP(hipMemcpyAsync(P->h_scal, P->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      FHS_HIP(hipStreamSynchronize(c->stream));
+      if (scalars)
+        for (int k = 0; k < 4; ++k) scalars[4 * it + k] = P->h_scal[k];
+      ++P->iters;
+    }
+    if (c->timing) {
+      c->collect();
+      for (auto& kv : c->totals) {
+        auto& t = P->ctx->totals[kv.first];
+        t.first += kv.second.first;
+        t.second += kv.second.second;
+      }
+      c->totals.clear();
+    }
+  });
+  return st;
+}
+
+int fhs_pdfb_norm_res(fhs_pdfb* P, double* norm_res) {
+  if (!P || !norm_res) return FHS_ERR_ARG;
+  return guard(P->ctx, [&] { data_get_norm_res(&P->own, P->D, norm_res); });
+}
+
+int fhs_pdfb_stream(fhs_pdfb* P, void** stream) {
+  if (!P || !stream) return FHS_ERR_ARG;
+  *stream = (void*)P->own.stream;
+  return FHS_OK;
+}
+
+}  // extern "C"

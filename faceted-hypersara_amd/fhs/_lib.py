@@ -44,6 +44,41 @@ PROTOTYPES = {
     'fhs_apply_scaled_fourier_transform': [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_apply_scaled_fourier_adjoint': [c_void_p, c_void_p, c_void_p, c_int, c_float_p, c_void_p],
     'fhs_proj_l2ball': [c_void_p, c_void_p, c_double, c_void_p, c_i64, c_void_p],
+    'fhs_data_create': [c_void_p, c_void_p, c_int, C.POINTER(c_void_p), c_int, c_i64_p, c_int_p,
+                        C.POINTER(c_void_p)],
+    'fhs_data_destroy': [c_void_p],
+    'fhs_data_init_fx_old': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
+    'fhs_update_dual_data_fidelity': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_double_p, c_int, c_int, c_double, c_float_p, c_int,
+                                      c_void_p, c_void_p, c_double_p, c_double_p, c_double_p],
+    'fhs_sara_create_host': [c_void_p, c_int, c_int, c_int, c_int, c_int_p, c_double_p, c_int,
+                             C.POINTER(c_void_p)],
+    'fhs_sara_destroy': [c_void_p],
+    'fhs_sara_info': [c_void_p, c_i64_p],
+    'fhs_sara_analysis': [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int],
+    'fhs_sara_synthesis': [c_void_p, c_void_p, c_void_p, c_int, C.c_float, c_void_p],
+    'fhs_update_dual_sparsity': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
+                                 C.c_float, c_void_p],
+    'fhs_compute_sara_prior': [c_void_p, c_void_p, c_void_p, c_int, c_double_p],
+    'fhs_lowrank_create_host': [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int_p, c_int_p, c_int_p,
+                                c_int_p, c_float_p, C.POINTER(c_void_p)],
+    'fhs_lowrank_destroy': [c_void_p],
+    'fhs_lowrank_info': [c_void_p, c_i64_p],
+    'fhs_update_dual_lowrankness': [c_void_p, c_void_p, c_void_p, c_void_p, c_double_p, c_double,
+                                    C.c_float, c_void_p],
+    'fhs_lowrank_singular_values': [c_void_p, c_void_p, c_void_p, c_double_p],
+    'fhs_nuclear_norm': [c_void_p, c_void_p, c_int, c_int, c_int, c_double_p],
+    'fhs_pdfb_create': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(c_void_p)],
+    'fhs_pdfb_destroy': [c_void_p],
+    'fhs_pdfb_set_params': [c_void_p, c_double_p, c_double, C.c_float, C.c_float, c_float_p, c_double_p,
+                            c_int, c_int, c_double, c_int],
+    'fhs_pdfb_set_graph': [c_void_p, c_int],
+    'fhs_pdfb_iterate': [c_void_p, c_int, c_double_p],
+    'fhs_pdfb_norm_res': [c_void_p, c_double_p],
+    'fhs_pdfb_stream': [c_void_p, C.POINTER(c_void_p)],
+    'fhs_data_forward': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    'fhs_data_adjoint': [c_void_p, c_void_p, c_void_p, c_void_p, c_float_p, c_void_p],
+    'fhs_sara_update_weights': [c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p],
     'fhs_update_primal': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                           c_double_p, c_double_p],
 }

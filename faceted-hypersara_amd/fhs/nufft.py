@@ -1,0 +1,102 @@
+"""Measurement-operator construction: min-max NUFFT interpolation (host, FP64).
+
+Builds, per channel, the scaling factors of the scaled FFT ``A`` and the
+dense-window interpolation matrix ``G`` (anchor + Ky*Kx coefficients per
+visibility) that ``op_p_nufft([{v} {u}], [Ny Nx], [Ky Kx], [oy*Ny ox*Nx],
+[Ny/2 Nx/2], ...)`` returns in the reference
+(``lib/operators/util_gen_measurement_operator_synth.m:85``).
+
+Min-max interpolator (Fessler & Sutton, IEEE TSP 2003): with scaling factors
+``s_n = sum_l a_l cos(gamma l (n - n0))`` (least-squares fit of the inverse
+Kaiser-Bessel transform), the J coefficients of frequency w are
+``T r(w)`` where ``T^-1_jk = sum_n s_n^2 cos(gamma (j-k)(n-n0))`` and
+``r_j(w) = sum_l a_l/2 [D(t_j + gamma l) + D(t_j - gamma l)]``,
+``t_j = gamma k_j - w``, ``D`` the Dirichlet kernel; phases move the
+origin to pixel N/2.  The coefficients are computed here once per
+visibility set; the iteration only reads them.
+"""
+import numpy as np
+
+from .core import GMatrix, NufftPlan
+
+KB_ALPHA_PER_J = 2.34
+SCALING_TERMS = 6
+
+
+class AxisInterp:
+    """1-D min-max interpolation setup for image size N on grid K."""
+
+    def __init__(self, N, K, J):
+        self.N, self.K, self.J = N, K, J
+        self.gamma = 2 * np.pi / K
+        self.n0 = (N - 1) / 2.0
+        n = np.arange(N) - self.n0
+        # inverse Kaiser-Bessel transform, then cosine-series fit
+        a_kb = KB_ALPHA_PER_J * J
+        arg = np.sqrt((a_kb ** 2 - (np.pi * J * n / K) ** 2).astype(complex))
+        target = 1.0 / np.real(np.sinh(arg) / arg)
+        C = np.cos(self.gamma * np.outer(n, np.arange(SCALING_TERMS)))
+        a = np.linalg.lstsq(C, target, rcond=None)[0]
+        self.alpha = a / a[0]
+        self.s = C @ self.alpha
+        d = np.subtract.outer(np.arange(J), np.arange(J))
+        gram = np.cos(self.gamma * d[..., None] * n) @ (self.s ** 2)
+        self.T = np.linalg.inv(gram)
+
+    def _dirichlet(self, t):
+        half = np.sin(0.5 * t)
+        out = np.full(t.shape, float(self.N))
+        ok = np.abs(half) >= 1e-12
+        out[ok] = np.sin(0.5 * self.N * t[ok]) / half[ok]
+        return out
+
+    def coefficients(self, w):
+        """(first grid index mod K, (M, J) complex coefficients) for frequencies w."""
+        w = np.asarray(w, dtype=np.float64)
+        k0 = np.floor(w / self.gamma - 0.5 * self.J)
+        k = k0[:, None] + np.arange(1, self.J + 1)
+        t = self.gamma * k - w[:, None]
+        r = self.alpha[0] * self._dirichlet(t)
+        for l in range(1, len(self.alpha)):
+            r += 0.5 * self.alpha[l] * (self._dirichlet(t + self.gamma * l) + self._dirichlet(t - self.gamma * l))
+        c = r @ self.T   # T symmetric
+        phase = np.exp(1j * (self.gamma * k * self.n0 - w[:, None] * (self.n0 - self.N // 2)))
+        return np.mod(k0 + 1, self.K).astype(np.int64), c * phase
+
+
+class Nufft2D:
+    """Scaled-FFT plan and per-channel G builder for an Ny x Nx image."""
+
+    def __init__(self, Ny, Nx, Ky=7, Kx=7, oy=2, ox=2):
+        self.Ny, self.Nx, self.Ky, self.Kx = Ny, Nx, Ky, Kx
+        self.Kty, self.Ktx = oy * Ny, ox * Nx
+        self.No = self.Kty * self.Ktx
+        self.ay = AxisInterp(Ny, self.Kty, Ky)
+        self.ax = AxisInterp(Nx, self.Ktx, Kx)
+
+    def window(self, u, v, row_scale=None):
+        """anchor (M,), coef (M, Ky*Kx) with tap jx*Ky + jy; G(m, :) scaled by row_scale."""
+        ay, cy = self.ay.coefficients(v)
+        ax, cx = self.ax.coefficients(u)
+        anchor = ay + self.Kty * ax
+        coef = (cx[:, :, None] * cy[:, None, :]).reshape(len(u), -1)
+        if row_scale is not None:
+            coef = coef * np.asarray(row_scale)[:, None]
+        return anchor, coef
+
+    def plan(self, ctx):
+        return NufftPlan(ctx, self.Ny, self.Nx, self.Kty, self.Ktx, self.ay.s, self.ax.s)
+
+    def gmatrix(self, ctx, u, v, row_scale=None):
+        anchor, coef = self.window(u, v, row_scale)
+        return GMatrix.from_window(ctx, anchor, coef, self.Kty, self.Ktx, self.Ky, self.Kx)
+
+
+def uniform_weights(u, v, Kty, Ktx):
+    """Preconditioning weights aW = 1/(number of uv points in the same cell of
+    the oversampled grid): util_gen_preconditioning_matrix with
+    gen_uniform_weight_matrix = true (``imaging/default_parameters.json:47-51``)."""
+    iy = np.floor((np.asarray(v) + np.pi) * Kty / (2 * np.pi)).astype(np.int64) % Kty
+    ix = np.floor((np.asarray(u) + np.pi) * Ktx / (2 * np.pi)).astype(np.int64) % Ktx
+    cell = iy + Kty * ix
+    return 1.0 / np.bincount(cell, minlength=Kty * Ktx)[cell]

@@ -39,6 +39,7 @@ typedef struct fhs_gmat fhs_gmat;     /* device interpolation matrix G (one chan
 typedef struct fhs_nufft fhs_nufft;   /* oversampled scaled FFT plan (A / At) */
 typedef struct fhs_sara fhs_sara;     /* SARA dictionary plan (Psit / Psi) */
 typedef struct fhs_data fhs_data;     /* data-fidelity state of L channels */
+typedef struct fhs_lowrank fhs_lowrank; /* facet layout of the nuclear-norm prior */
 
 /* ------------------------------------------------------------ context -- */
 int fhs_ctx_create(int device, fhs_ctx** out);
@@ -113,6 +114,157 @@ int fhs_proj_l2ball(fhs_ctx* ctx, const void* x, double eps, const void* y,
 int fhs_update_primal(fhs_ctx* ctx, float* xsol, const float* g_a,
                       const float* g_b, const float* g_c, float* xhat,
                       int64_t n, double* rel_x, double* norm_x);
+
+/* --------------------------------------------------- data fidelity -- */
+/* State of the data term of L channels sharing one scaled FFT A.  G[l]
+ * (rows of channel l; its columns index A's Kty*Ktx grid, i.e. W is the
+ * whole grid) is stacked: channel l owns rows row_offset[l]..row_offset[l+1]
+ * of every M-vector (v2, y, proj, pU, Gx_old, Lambda).  Data blocks
+ * (y{l}{b} of the reference) are row ranges block_start[b]..block_start[b+1]
+ * of channel block_channel[b]; they tile all rows in channel order.
+ * Host arrays; G and A must outlive the state. */
+int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* const* G,
+                    int nblocks, const int64_t* block_start, const int* block_channel,
+                    fhs_data** out);
+int fhs_data_destroy(fhs_data* data);
+/* Gx_old = G A(x) (Lambda (G + G') A(x) for the DR form): the state that
+ * replaces Fx_old = A(x) (src/hs/hyperSARA.m, Fxi_old = apply_scaled_
+ * fourier_transform(xsol, A, No)); by linearity G(2 Fx - Fx_old) =
+ * 2 G Fx - Gx_old, so M complex per channel are kept instead of No. */
+int fhs_data_init_fx_old(fhs_ctx* ctx, fhs_data* data, const float* x, const float* Lambda,
+                         int flag_dimensionality_reduction, void* Gx_old);
+/* y = Lambda .* (G_l A x_l) for every channel (stacked): the measurement
+ * operator of HS_forward_operator_G.m / util_operator_norm.m:45 (Lambda
+ * NULL = 1, or sqrt(aW) for the preconditioned operator). */
+int fhs_data_forward(fhs_ctx* ctx, fhs_data* data, const float* x, const float* Lambda, void* y);
+/* out_l = scale_l real(At(G_l' (Lambda .* y))): HS_adjoint_operator_G.m
+ * (scale: host float[L] or NULL). */
+int fhs_data_adjoint(fhs_ctx* ctx, fhs_data* data, const void* y, const float* Lambda,
+                     const float* scale, float* out);
+/* [v2, Ftx, Fx_old, proj, norm_res, global_norm_res, norm_epsilon] =
+ *   update_dual_data_fidelity(v2, y, x, Fx_old, proj, A, At, G, W, pU,
+ *     epsilon, elipse_proj_max_iter, elipse_proj_min_iter, elipse_proj_eps,
+ *     sigma22, flag_dimensionality_reduction, Lambda)
+ * src/update_dual_data_fidelity.m:1-149.  Device: v2, y, proj, Gx_old
+ * (complex[Mtot], updated in place), x float[L][Nx][Ny], pU float[Mtot] (NULL
+ * = 1), Lambda float[Mtot] (DR only, NULL = 1), Ftx float[L][Nx][Ny] out.
+ * Host: epsilon double[nblocks], sigma22 float[L], norm_res double[nblocks]
+ * out (may be NULL), global_norm_res and norm_epsilon (sums of squares) out. */
+int fhs_update_dual_data_fidelity(fhs_ctx* ctx, fhs_data* data, void* v2, const void* y,
+                                  const float* x, void* Gx_old, void* proj, const float* pU,
+                                  const double* epsilon, int elipse_proj_max_iter,
+                                  int elipse_proj_min_iter, double elipse_proj_eps,
+                                  const float* sigma22, int flag_dimensionality_reduction,
+                                  const float* Lambda, float* Ftx, double* norm_res,
+                                  double* global_norm_res, double* norm_epsilon);
+
+/* ------------------------------------------------------ SARA prior -- */
+/* SARA dictionary (op_sp_wlt_basis of the SARA-dictionary submodule,
+ * built at imaging/imaging.m:797-810 and src/hs/hyperSARA.m): nbases
+ * Daubechies bases (reconstruction low-pass filters lo_r concatenated,
+ * lengths filter_len[b], even, <= 16) + the Dirac basis when with_dirac,
+ * nlevel levels, 'zpd' extension, each basis scaled by 1/sqrt(P).
+ * Per-channel coefficient vector: MATLAB wavedec2 order per basis, bases in
+ * order, Dirac last. */
+int fhs_sara_create_host(fhs_ctx* ctx, int Ny, int Nx, int nlevel, int nbases,
+                         const int* filter_len, const double* lo_r, int with_dirac,
+                         fhs_sara** out);
+int fhs_sara_destroy(fhs_sara* S);
+int fhs_sara_info(const fhs_sara* S, int64_t* n_coefs);
+/* coefs(:, l) (+)= Psit(x(:, :, l)); coefs float[L][s] */
+int fhs_sara_analysis(fhs_ctx* ctx, const fhs_sara* S, const float* x, int L, float* coefs,
+                      int accumulate);
+/* out(:, :, l) = scale * Psi(coefs(:, l)) */
+int fhs_sara_synthesis(fhs_ctx* ctx, const fhs_sara* S, const float* coefs, int L, float scale,
+                       float* out);
+/* [v1, g1] = hs_update_dual_sparsity_distributed(v1, Psit, Psi, xhat,
+ *   weights1, beta1, sigma1)  src/hs/hs_update_dual_sparsity_distributed.m:4-17
+ * (also fhs_update_dual_sparsity.m:67-93 on the whole image, and the L = 1
+ * SARA step src/sara/sara.m:921-926).  v1 float[L][s] in place, xhat
+ * float[L][Nx][Ny], weights1 float[s] (NULL = 1), g1 out. */
+int fhs_update_dual_sparsity(fhs_ctx* ctx, const fhs_sara* S, float* v1, const float* xhat, int L,
+                             const float* weights1, double beta1, float sigma1, float* g1);
+/* weights1 = upsilon ./ (upsilon + ||Psit(x)||_row): the l21 reweighting
+ * of src/hs/hs_update_weights_sparsity_distributed.m:8-13 and
+ * src/fhs/fhs_update_weights.m:82-92 (upsilon = sig * alpha); float[s] out. */
+int fhs_sara_update_weights(fhs_ctx* ctx, const fhs_sara* S, const float* x, int L, double upsilon,
+                            float* weights1);
+/* f = compute_sara_prior(x, Psit, n_wavelet_coefficients)
+ * src/compute_sara_prior.m:33-41 (host double out). */
+int fhs_compute_sara_prior(fhs_ctx* ctx, const fhs_sara* S, const float* x, int L, double* out);
+
+/* -------------------------------------------------- low-rank prior -- */
+/* Facets q = qx*Qy + qy (src/fhs/facetHyperSARA.m:310-321 order) with origin
+ * (oy, ox) and size (py, px); windows: concatenated py*px apodisation
+ * windows (column-major) or NULL (= 1).  L <= 16.  A single facet covering
+ * the image is the HyperSARA global prior.  v0 is stored per facet as
+ * L x (py*px) blocks, facet after facet (total v0_size floats). */
+int fhs_lowrank_create_host(fhs_ctx* ctx, int Ny, int Nx, int L, int Qy, int Qx, const int* oy,
+                            const int* ox, const int* py, const int* px, const float* windows,
+                            fhs_lowrank** out);
+int fhs_lowrank_destroy(fhs_lowrank* R);
+int fhs_lowrank_info(const fhs_lowrank* R, int64_t* v0_size);
+/* [v0, g0] = fhs_update_dual_lowrankness(v0, xhat, apodization_window,
+ *   weights0, beta0)   src/fhs/fhs_update_dual_lowrankness.m:40-45, and
+ * hs_update_dual_lowrankness_serial.m:5-10 for one facet.  g0 =
+ * sigma0 * sum_q w_q .* v0_q (overlap-added image, float[L][Nx][Ny]).
+ * weights0: host double[Q][L] (descending-singular-value order). */
+int fhs_update_dual_lowrankness(fhs_ctx* ctx, fhs_lowrank* R, float* v0, const float* xhat,
+                                const double* weights0, double beta0, float sigma0, float* g0);
+/* singular values (descending) of w_q .* x_q per facet: host double[Q][L] */
+int fhs_lowrank_singular_values(fhs_ctx* ctx, fhs_lowrank* R, const float* x, double* sv);
+/* f = nuclear_norm(x)   src/nuclear_norm.m:25-26; x float[L][Nx][Ny] */
+int fhs_nuclear_norm(fhs_ctx* ctx, const float* x, int Ny, int Nx, int L, double* out);
+
+/* ------------------------------------------------------ PDFB engine -- */
+/* Whole PDFB iterations of SARA (lowrank = NULL, L = 1), HyperSARA (one
+ * facet) and Faceted HyperSARA (Q facets): src/sara/sara.m:521-633,
+ * src/hs/hyperSARA.m:565-617, src/fhs/facetHyperSARA.m:701-783.
+ * Caller-owned device buffers (float = float32, complex = complex64):
+ *   x, xhat, g0 (NULL without low-rank prior), g1, g2: float[L][Nx][Ny]
+ *   v0: lowrank v0_size floats (NULL without), v1: float[L][s]
+ *   v2, proj, Gx_old: complex[Mtot] (in/out), y: complex[Mtot]
+ *   pU: float[Mtot] (NULL = 1), weights1: float[s] (NULL = 1),
+ *   Lambda: float[Mtot] (DR only, NULL = 1).
+ * The engine runs on its own stream (default flags and priority). */
+typedef struct fhs_pdfb fhs_pdfb;
+typedef struct {
+  int Ny, Nx;
+  float* x;
+  float* xhat;
+  float* g0;
+  float* g1;
+  float* g2;
+  float* v0;
+  float* v1;
+  void* v2;
+  void* proj;
+  void* Gx_old;
+  const void* y;
+  const float* pU;
+  const float* weights1;
+  const float* Lambda;
+} fhs_pdfb_buffers;
+int fhs_pdfb_create(fhs_ctx* ctx, fhs_data* data, fhs_sara* sara, fhs_lowrank* lowrank,
+                    const fhs_pdfb_buffers* bufs, fhs_pdfb** out);
+int fhs_pdfb_destroy(fhs_pdfb* p);
+/* thresh0: host double[Q][L] = beta0_q * weights0_q (descending singular
+ * value order; NULL without low-rank prior); beta1 = gamma / sigma1
+ * (src/fhs/facetHyperSARA.m:598-603); sigma00 = tau sigma0, sigma11 = tau
+ * sigma1, sigma22: host float[L] = tau sigma2 (:582-596);
+ * epsilon: host double[nblocks]. */
+int fhs_pdfb_set_params(fhs_pdfb* p, const double* thresh0, double beta1, float sigma00, float sigma11,
+                        const float* sigma22, const double* epsilon, int elipse_proj_max_iter,
+                        int elipse_proj_min_iter, double elipse_proj_eps,
+                        int flag_dimensionality_reduction);
+/* replay iterations from a captured hipGraph (default on) */
+int fhs_pdfb_set_graph(fhs_pdfb* p, int enable);
+/* runs n_iter iterations; scalars: host double[4*n_iter] or NULL:
+ * sum (x_prev - x)^2, sum x^2, sum_b norm_res_b^2, sum_b eps_b^2 */
+int fhs_pdfb_iterate(fhs_pdfb* p, int n_iter, double* scalars);
+/* last iteration's per-block residual norms ||G_b A x - y_b||: host double[nblocks] */
+int fhs_pdfb_norm_res(fhs_pdfb* p, double* norm_res);
+int fhs_pdfb_stream(fhs_pdfb* p, void** stream);
 
 #ifdef __cplusplus
 }

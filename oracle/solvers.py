@@ -21,11 +21,14 @@ class OracleProblem:
     ranges; epsilon: per-channel list of block bounds.
     """
 
-    def __init__(self, Ny, Nx, u, v, y, pU, blocks, epsilon, nufft_kwargs=None):
+    def __init__(self, Ny, Nx, u, v, y, pU, blocks, epsilon, g_scale=None, nufft_kwargs=None):
         self.Ny, self.Nx = Ny, Nx
         self.L = len(u)
         kw = nufft_kwargs or {}
         self.ops = [NufftOracle(u[l], v[l], Ny, Nx, **kw) for l in range(self.L)]
+        if g_scale is not None:     # whitening G <- G / sigma (util_gen_measurements_snr_new.m:78)
+            for l, op in enumerate(self.ops):
+                op.G = op.G * g_scale[l]
         self.No = self.ops[0].No
         self.A = [op.A for op in self.ops]
         self.At = [op.At for op in self.ops]
