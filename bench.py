@@ -151,11 +151,9 @@ def main():
         for l in range(prob.L):
             b = np.concatenate([[0], np.cumsum(prob.blocks[l])])
             blocks.append([(int(b[i]), int(b[i + 1])) for i in range(len(prob.blocks[l]))])
-        allp = meas.pU.cpu().numpy().astype(np.float64)
-        off = np.concatenate([[0], np.cumsum([len(u) for u in prob.u])])
-        pU = [allp[off[l]:off[l + 1]] for l in range(prob.L)]
+        pU = [np.asarray(p, dtype=np.float64) for p in meas.pU_host]
         with threadpool_limits(limits=1):
-            op = OracleProblem(prob.Ny, prob.Nx, prob.u, prob.v, prob.y, pU, blocks, prob.epsilon,
+            op = OracleProblem(prob.Ny, prob.Nx, meas.u, meas.v, meas.y_host, pU, blocks, prob.epsilon,
                                g_scale=[1.0 / s for s in prob.sigma_noise])
             sig, sig_bar = solvers.noise_levels_hs(prob.Ny, prob.Nx, prob.L, np.ones(prob.L), son)
             pr = dict(solvers.DEFAULT_PARAM)

@@ -24,6 +24,7 @@ struct fhs_data {
   int nblocks = 0;
   int64_t Mtot = 0;
   int maxM = 0;
+  int adj_grid = 0;                    // max adjoint units + grid-tail units over channels
   int64_t No = 0;
   const fhs_nufft* A = nullptr;
   std::vector<int64_t> row_off;        // [L+1]
@@ -189,30 +190,46 @@ __global__ __launch_bounds__(FB_THREADS) void k_l2_dual(const int64_t* __restric
   if (threadIdx.x == 0) norm_res[blockIdx.x] = acc[1];
 }
 
-__global__ __launch_bounds__(256) void k_data_adjoint(const GDev* __restrict__ gd,
-                                                      const int64_t* __restrict__ row_off,
-                                                      const float2* __restrict__ v2,
-                                                      const float* __restrict__ lam, int dr,
-                                                      int64_t No, float2* __restrict__ g2) {
+// g2_l = G_l' (lam .* v2_l) (+ G_l (lam .* v2_l) in the DR form), one
+// workgroup per adjoint work unit of channel blockIdx.y; grid points past
+// a channel's columns are zero.
+__global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint(const GDev* __restrict__ gd,
+                                                              const int64_t* __restrict__ row_off,
+                                                              const float2* __restrict__ v2,
+                                                              const float* __restrict__ lam, int dr,
+                                                              int64_t No, float2* __restrict__ g2) {
+  __shared__ float2 prod[ADJ_LDS];
   const int l = blockIdx.y;
   const GDev g = gd[l];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= No) return;
-  float2 acc = make_float2(0.f, 0.f);
-  if (k < g.ncols) {
-    const float2* v = v2 + row_off[l];
-    const float* lm = lam ? lam + row_off[l] : nullptr;
-    if (lm) {
-      auto u = [&](int r) { return cscale(v[r], lm[r]); };
-      acc = g_col_dot(g, k, u);
-      if (dr) acc = cadd(acc, g_row_apply(g, k, u));
-    } else {
-      auto u = [&](int r) { return v[r]; };
-      acc = g_col_dot(g, k, u);
-      if (dr) acc = cadd(acc, g_row_apply(g, k, u));
+  float2* out = g2 + (size_t)l * No;
+  const int u = blockIdx.x;
+  if (u >= g.nunits) {
+    // zero the grid tail beyond the matrix columns (only when ncols < No)
+    const int64_t extra_units = (No - g.ncols + ADJ_COLS - 1) / ADJ_COLS;
+    const int64_t t = u - g.nunits;
+    if (t < extra_units) {
+      const int64_t c0 = g.ncols + t * ADJ_COLS;
+      for (int64_t c = c0 + threadIdx.x; c < c0 + ADJ_COLS && c < No; c += ADJ_THREADS)
+        out[c] = make_float2(0.f, 0.f);
     }
+    return;
   }
-  g2[(size_t)l * No + k] = acc;
+  const float2* v = v2 + row_off[l];
+  const float* lm = lam ? lam + row_off[l] : nullptr;
+  auto none = [](int) { return make_float2(0.f, 0.f); };
+  if (lm) {
+    auto w = [&](int r) { return cscale(v[r], lm[r]); };
+    if (dr)
+      g_adjoint_unit(g, u, w, [&](int c) { return g_row_apply(g, c, w); }, out, prod);
+    else
+      g_adjoint_unit(g, u, w, none, out, prod);
+  } else {
+    auto w = [&](int r) { return v[r]; };
+    if (dr)
+      g_adjoint_unit(g, u, w, [&](int c) { return g_row_apply(g, c, w); }, out, prod);
+    else
+      g_adjoint_unit(g, u, w, none, out, prod);
+  }
 }
 
 // y = lam .* (G_l A x_l) for all channels (stacked rows)
@@ -282,8 +299,8 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
   }
   {
     Scope sc(ctx, "data_adjoint");
-    dim3 grid(cdiv(D->No, 256), L);
-    k_data_adjoint<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr, D->No, D->g2);
+    dim3 grid(D->adj_grid, L);
+    k_data_adjoint<<<grid, ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr, D->No, D->g2);
     check_launch();
   }
   nufft_adjoint(ctx, D->A, D->g2, L, D->d_sigma22, Ftx);
@@ -327,8 +344,8 @@ void data_adjoint(fhs_ctx* ctx, fhs_data* D, const float2* v, const float* lam, 
                   float* out) {
   {
     Scope sc(ctx, "data_adjoint");
-    dim3 grid(cdiv(D->No, 256), D->L);
-    k_data_adjoint<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, v, lam, 0, D->No, D->g2);
+    dim3 grid(D->adj_grid, D->L);
+    k_data_adjoint<<<grid, ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v, lam, 0, D->No, D->g2);
     check_launch();
   }
   nufft_adjoint(ctx, D->A, D->g2, D->L, scale_dev, out);
@@ -374,6 +391,8 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
       gd[l] = G[l]->d;
       D->row_off[l + 1] = D->row_off[l] + G[l]->M;
       D->maxM = std::max<int>(D->maxM, (int)G[l]->M);
+      const int64_t tail = (D->No - G[l]->ncols + ADJ_COLS - 1) / ADJ_COLS;
+      D->adj_grid = std::max<int>(D->adj_grid, G[l]->d.nunits + (int)tail);
       square = square && (G[l]->ncols == G[l]->M);
     }
     D->square = square;

@@ -36,6 +36,25 @@ void build_csc(int64_t ncols, const std::vector<int>& rows, const std::vector<in
   }
 }
 
+// column ranges of ~ADJ_ENTRIES nonzeros (<= ADJ_COLS columns)
+std::vector<int> adjoint_units(const std::vector<int>& colptr) {
+  const int64_t ncols = (int64_t)colptr.size() - 1;
+  std::vector<int> u{0};
+  int64_t cnt = 0, cols = 0;
+  for (int64_t c = 0; c < ncols; ++c) {
+    const int64_t len = colptr[c + 1] - colptr[c];
+    if (cols > 0 && (cnt + len > ADJ_ENTRIES || cols >= ADJ_COLS)) {
+      u.push_back((int)c);
+      cnt = 0;
+      cols = 0;
+    }
+    cnt += len;
+    ++cols;
+  }
+  u.push_back((int)ncols);
+  return u;
+}
+
 __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
                                const float* __restrict__ lam, float2* __restrict__ y) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,16 +64,15 @@ __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
   y[m] = acc;
 }
 
-__global__ void k_gmat_adjoint(GDev g, const float2* __restrict__ y,
-                               const float* __restrict__ lam, float2* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.ncols) return;
-  float2 acc;
+__global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint(GDev g, const float2* __restrict__ y,
+                                                               const float* __restrict__ lam,
+                                                               float2* __restrict__ out) {
+  __shared__ float2 prod[ADJ_LDS];
+  auto none = [](int) { return make_float2(0.f, 0.f); };
   if (lam)
-    acc = g_col_dot(g, k, [&](int r) { return cscale(y[r], lam[r]); });
+    g_adjoint_unit(g, blockIdx.x, [&](int r) { return cscale(y[r], lam[r]); }, none, out, prod);
   else
-    acc = g_col_dot(g, k, [&](int r) { return y[r]; });
-  out[k] = acc;
+    g_adjoint_unit(g, blockIdx.x, [&](int r) { return y[r]; }, none, out, prod);
 }
 
 }  // namespace
@@ -101,12 +119,16 @@ int fhs_gmat_create_csr_host(fhs_ctx* ctx, int64_t M, int64_t ncols, const int64
     std::vector<int> colptr, crow;
     std::vector<float2> cval;
     build_csc(ncols, rows, cols, v, colptr, crow, cval);
+    const std::vector<int> units = adjoint_units(colptr);
     FHS_HIP(hipSetDevice(ctx->device));
     fhs_gmat* g = new fhs_gmat();
     g->device = ctx->device;
     g->M = M;
     g->ncols = ncols;
     g->nnz = nnz;
+    g->mem[6] = upload(units);
+    g->d.ucol = (const int*)g->mem[6];
+    g->d.nunits = (int)units.size() - 1;
     g->mem[0] = upload(ell_c);
     g->mem[1] = upload(ell_v);
     g->mem[2] = upload(colptr);
@@ -162,12 +184,16 @@ int fhs_gmat_create_window_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, int K
     std::vector<int> colptr, crow;
     std::vector<float2> cval;
     build_csc(ncols, rows, cols, v, colptr, crow, cval);
+    const std::vector<int> units = adjoint_units(colptr);
     FHS_HIP(hipSetDevice(ctx->device));
     fhs_gmat* g = new fhs_gmat();
     g->device = ctx->device;
     g->M = M;
     g->ncols = ncols;
     g->nnz = nnz;
+    g->mem[6] = upload(units);
+    g->d.ucol = (const int*)g->mem[6];
+    g->d.nunits = (int)units.size() - 1;
     g->mem[0] = upload(anc);
     g->mem[1] = upload(wv);
     g->mem[2] = upload(colptr);
@@ -226,7 +252,7 @@ int fhs_apply_adjoint_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* y,
   return guard(ctx, [&] {
     FHS_CHECK(G && Fx && y, "apply_adjoint_operator: null argument");
     Scope sc(ctx, "gmat_adjoint");
-    k_gmat_adjoint<<<cdiv(G->ncols, 256), 256, 0, ctx->stream>>>(
+    k_gmat_adjoint<<<G->d.nunits, ADJ_THREADS, 0, ctx->stream>>>(
         G->d, (const float2*)y, Lambda, (float2*)Fx);
     check_launch();
   });

@@ -30,7 +30,17 @@ struct GDev {
   const int* colptr;    // [ncols + 1]
   const int* csc_rows;  // [nnz]
   const float2* csc_vals;
+  const int* ucol;      // adjoint work units: unit u = columns [ucol[u], ucol[u+1])
+  int nunits;
 };
+
+// Adjoint work units: contiguous column ranges holding about ADJ_ENTRIES
+// nonzeros (and at most ADJ_COLS columns), so every workgroup streams the
+// same amount of CSC data whatever the uv density.
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_ENTRIES = 2048;
+constexpr int ADJ_COLS = 2048;
+constexpr int ADJ_LDS = 4096;
 
 // sum over taps j of row m: G[m, c_j] * f(c_j)
 template <class F>
@@ -83,11 +93,39 @@ __device__ __forceinline__ float2 g_col_dot(const GDev& g, int k, W&& w) {
   return acc;
 }
 
+// One adjoint work unit: the unit's CSC entries are loaded coalesced (lane
+// t takes entries t, t+256, ...), their products conj(G) w(row) staged in
+// LDS, then each thread sums whole columns from LDS and writes them: every
+// output column belongs to exactly one unit (no atomics).  extra(c) is added
+// to column c (the G w term of the Hermitian DR form).
+template <class W, class X>
+__device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& extra,
+                                               float2* __restrict__ out, float2* prod) {
+  const int c0 = g.ucol[u], c1 = g.ucol[u + 1];
+  const int e0 = g.colptr[c0], n = g.colptr[c1] - e0;
+  if (n <= ADJ_LDS) {
+    for (int e = threadIdx.x; e < n; e += ADJ_THREADS) {
+      const float2 v = g.csc_vals[e0 + e];
+      const float2 y = w(g.csc_rows[e0 + e]);
+      prod[e] = make_float2(v.x * y.x + v.y * y.y, v.x * y.y - v.y * y.x);
+    }
+    __syncthreads();
+    for (int c = c0 + threadIdx.x; c < c1; c += ADJ_THREADS) {
+      const int a = g.colptr[c] - e0, b = g.colptr[c + 1] - e0;
+      float2 s = make_float2(0.f, 0.f);
+      for (int e = a; e < b; ++e) s = cadd(s, prod[e]);
+      out[c] = cadd(s, extra(c));
+    }
+  } else {   // a column longer than the LDS stage: direct gather
+    for (int c = c0 + threadIdx.x; c < c1; c += ADJ_THREADS) out[c] = cadd(g_col_dot(g, c, w), extra(c));
+  }
+}
+
 }  // namespace fhs
 
 struct fhs_gmat {
   int device = 0;
   fhs::GDev d{};
   int64_t M = 0, ncols = 0, nnz = 0;
-  void* mem[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* mem[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };

@@ -41,6 +41,9 @@ int lowrank_channels(const fhs_lowrank* R);
 struct fhs_pdfb {
   fhs_ctx* ctx = nullptr;
   fhs_ctx own{};            // private context state bound to the engine stream
+  fhs_ctx side{};           // second stream: the prior steps run beside the data step
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool concurrent = true;
   fhs_data* D = nullptr;
   fhs_sara* S = nullptr;
   fhs_lowrank* R = nullptr;
@@ -63,14 +66,36 @@ struct fhs_pdfb {
 namespace fhs {
 namespace {
 
+// After the primal update the two prior steps (reading xhat) and the data
+// step (reading x) are independent: with `concurrent` they are forked onto
+// the side stream and joined before the next iteration's primal update.
 void run_iteration(fhs_pdfb* P) {
   fhs_ctx* c = &P->own;
+  fhs_ctx* p = P->concurrent ? &P->side : c;
   const auto& b = P->b;
   primal_update(c, b.x, b.g1, b.g2, b.g0, b.xhat, P->N * P->L, P->d_scal);
-  if (P->R) lowrank_dual_step(c, P->R, b.v0, b.xhat, P->sigma00, b.g0);
-  sara_dual_step(c, P->S, b.v1, b.xhat, P->L, b.weights1, P->beta1, P->sigma11, b.g1);
+  if (P->concurrent) {
+    FHS_HIP(hipEventRecord(P->ev_fork, c->stream));
+    FHS_HIP(hipStreamWaitEvent(p->stream, P->ev_fork, 0));
+  }
+  if (P->R) lowrank_dual_step(p, P->R, b.v0, b.xhat, P->sigma00, b.g0);
+  sara_dual_step(p, P->S, b.v1, b.xhat, P->L, b.weights1, P->beta1, P->sigma11, b.g1);
   data_step(c, P->D, (float2*)b.v2, (const float2*)b.y, b.x, (float2*)b.Gx_old, (float2*)b.proj, b.pU,
             P->fb_max, P->fb_min, P->fb_eps, P->dr, b.Lambda, b.g2, P->d_scal + 2);
+  if (P->concurrent) {
+    FHS_HIP(hipEventRecord(P->ev_join, p->stream));
+    FHS_HIP(hipStreamWaitEvent(c->stream, P->ev_join, 0));
+  }
+}
+
+void merge_timing(fhs_ctx* from, fhs_ctx* into) {
+  from->collect();
+  for (auto& kv : from->totals) {
+    auto& t = into->totals[kv.first];
+    t.first += kv.second.first;
+    t.second += kv.second.second;
+  }
+  from->totals.clear();
 }
 
 void drop_graph(fhs_pdfb* P) {
@@ -102,6 +127,10 @@ int fhs_pdfb_create(fhs_ctx* ctx, fhs_data* D, fhs_sara* S, fhs_lowrank* R,
     P->own.device = ctx->device;
     FHS_HIP(hipSetDevice(ctx->device));
     FHS_HIP(hipStreamCreate(&P->own.stream));
+    FHS_HIP(hipStreamCreate(&P->side.stream));
+    P->side.device = ctx->device;
+    FHS_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+    FHS_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
     P->D = D;
     P->S = S;
     P->R = R;
@@ -124,9 +153,15 @@ int fhs_pdfb_destroy(fhs_pdfb* P) {
   drop_graph(P);
   (void)hipFree(P->d_scal);
   (void)hipHostFree(P->h_scal);
+  (void)hipStreamSynchronize(P->side.stream);
   P->own.ws.release();
+  P->side.ws.release();
   for (auto e : P->own.event_pool) (void)hipEventDestroy(e);
+  for (auto e : P->side.event_pool) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(P->ev_fork);
+  (void)hipEventDestroy(P->ev_join);
   (void)hipStreamDestroy(P->own.stream);
+  (void)hipStreamDestroy(P->side.stream);
   delete P;
   return FHS_OK;
 }
@@ -174,6 +209,13 @@ int fhs_pdfb_set_graph(fhs_pdfb* P, int enable) {
   return FHS_OK;
 }
 
+int fhs_pdfb_set_concurrent(fhs_pdfb* P, int enable) {
+  if (!P) return FHS_ERR_ARG;
+  if ((enable != 0) != P->concurrent) drop_graph(P);
+  P->concurrent = enable != 0;
+  return FHS_OK;
+}
+
 int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
   if (!P) return FHS_ERR_ARG;
   fhs_ctx* c = &P->own;
@@ -181,6 +223,7 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
     FHS_CHECK(P->params_set, "pdfb_iterate: call fhs_pdfb_set_params first");
     FHS_CHECK(n_iter >= 1, "pdfb_iterate: n_iter >= 1");
     c->timing = P->ctx->timing;
+    P->side.timing = P->ctx->timing;
     for (int it = 0; it < n_iter; ++it) {
       const bool graph_now = P->use_graph && !c->timing && P->iters >= 1;
       if (graph_now && !P->exec) {
@@ -200,13 +243,8 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
       ++P->iters;
     }
     if (c->timing) {
-      c->collect();
-      for (auto& kv : c->totals) {
-        auto& t = P->ctx->totals[kv.first];
-        t.first += kv.second.first;
-        t.second += kv.second.second;
-      }
-      c->totals.clear();
+      merge_timing(c, P->ctx);
+      merge_timing(&P->side, P->ctx);
     }
   });
   return st;

@@ -11,6 +11,7 @@ from .operators import (  # noqa: F401
     apply_scaled_fourier_transform,
     fhs_update_primal,
     proj_l2ball,
+    update_epsilon,
 )
 from .data_fidelity import DataOperator, update_dual_data_fidelity  # noqa: F401,E402
 from .lowrank import LowRankPrior, hs_update_dual_lowrankness_serial, nuclear_norm  # noqa: F401,E402

@@ -34,7 +34,14 @@ PROTOTYPES = {
     'fhs_gmat_create_csr_host': [c_void_p, c_i64, c_i64, c_i64_p, c_i64_p, c_float_p, C.POINTER(c_void_p)],
     'fhs_gmat_create_window_host': [c_void_p, c_i64, c_int, c_int, c_int, c_int, c_i32_p, c_float_p,
                                     C.POINTER(c_void_p)],
+    'fhs_gmat_create_csc_host': [c_void_p, c_i64, c_i64, c_i64_p, c_i64_p, c_float_p, C.POINTER(c_void_p)],
     'fhs_gmat_destroy': [c_void_p],
+    'fhs_device_alloc': [c_void_p, c_i64, C.POINTER(c_void_p)],
+    'fhs_device_free': [c_void_p, c_void_p],
+    'fhs_memcpy_h2d': [c_void_p, c_void_p, c_void_p, c_i64],
+    'fhs_memcpy_d2h': [c_void_p, c_void_p, c_void_p, c_i64],
+    'fhs_update_epsilon': [c_double_p, c_int, c_int_p, c_double_p, c_int, c_double, c_double, c_int, c_double,
+                           c_int_p],
     'fhs_gmat_info': [c_void_p, c_i64_p, c_i64_p, c_i64_p],
     'fhs_apply_direct_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_apply_adjoint_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
@@ -73,6 +80,7 @@ PROTOTYPES = {
     'fhs_pdfb_set_params': [c_void_p, c_double_p, c_double, C.c_float, C.c_float, c_float_p, c_double_p,
                             c_int, c_int, c_double, c_int],
     'fhs_pdfb_set_graph': [c_void_p, c_int],
+    'fhs_pdfb_set_concurrent': [c_void_p, c_int],
     'fhs_pdfb_iterate': [c_void_p, c_int, c_double_p],
     'fhs_pdfb_norm_res': [c_void_p, c_double_p],
     'fhs_pdfb_stream': [c_void_p, C.POINTER(c_void_p)],

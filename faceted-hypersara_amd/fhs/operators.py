@@ -94,3 +94,32 @@ def fhs_update_primal(xsol, g, *more_g):
                                       _ptr(xhat), xsol.numel(), C.byref(rel), C.byref(nrm)),
               'update_primal')
     return xsol, xhat, rel.value, nrm.value
+
+
+def update_epsilon(epsilon, t, t_block, norm_res, adapt_eps_tol_in, adapt_eps_tol_out,
+                   adapt_eps_steps, adapt_eps_change_percentage):
+    """``[epsilon, t_block] = update_epsilon(...)`` — ``src/update_epsilon.m:1-73``.
+
+    Cells ``{L}{nblocks}`` as nested lists (or flat arrays); returns updated
+    copies in the same nesting."""
+    import numpy as np
+    nested = isinstance(epsilon, (list, tuple)) and len(epsilon) and isinstance(epsilon[0], (list, tuple, np.ndarray))
+    shape = [len(e) for e in epsilon] if nested else None
+    flat = lambda c: np.ascontiguousarray(np.concatenate([np.asarray(e, dtype=float) for e in c]) if nested
+                                          else np.asarray(c, dtype=float))  # noqa: E731
+    eps = flat(epsilon)
+    tb = np.ascontiguousarray(flat(t_block).astype(np.int32))
+    nr = flat(norm_res)
+    n_upd = C.c_int()
+    st = lib().fhs_update_epsilon(eps.ctypes.data_as(C.POINTER(C.c_double)), int(t),
+                                  tb.ctypes.data_as(C.POINTER(C.c_int)), nr.ctypes.data_as(C.POINTER(C.c_double)),
+                                  len(eps), float(adapt_eps_tol_in), float(adapt_eps_tol_out), int(adapt_eps_steps),
+                                  float(adapt_eps_change_percentage), C.byref(n_upd))
+    if st != 0:
+        raise ValueError('update_epsilon: bad argument')
+    if not nested:
+        return eps, tb
+    out_e, out_t, i = [], [], 0
+    for n in shape:
+        out_e.append(list(eps[i:i + n])); out_t.append(list(tb[i:i + n])); i += n
+    return out_e, out_t

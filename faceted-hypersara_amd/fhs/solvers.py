@@ -50,7 +50,7 @@ class Measurement:
     If the problem carries no data yet, noiseless visibilities are computed
     with the GPU operator and noise is added (``synth.add_noise``)."""
 
-    def __init__(self, ctx, problem, precond=True):
+    def __init__(self, ctx, problem, precond=True, sort_rows=True):
         self.ctx, self.problem = ctx, problem
         P = problem
         self.L, self.Ny, self.Nx = P.L, P.Ny, P.Nx
@@ -62,14 +62,34 @@ class Measurement:
             y0 = self._split(Draw, self._forward(Draw, _cube(P.X0)).cpu().numpy().astype(np.complex128))
             del Draw, Graw
             add_noise(P, y0)
-        self.G = [self.nufft.gmatrix(ctx, P.u[l], P.v[l], row_scale=np.full(len(P.u[l]), 1.0 / P.sigma_noise[l]))
+        # Visibilities are reordered inside each block by the grid cell of
+        # their interpolation window: neighbouring rows then gather
+        # neighbouring grid points (forward) and the rows of a grid column
+        # sit close together (adjoint).  Per-block results are unchanged.
+        self.order = []
+        for l in range(P.L):
+            if sort_rows:
+                ay = np.mod(np.floor(P.v[l] / self.nufft.ay.gamma - 0.5 * self.nufft.Ky) + 1, self.nufft.Kty)
+                ax = np.mod(np.floor(P.u[l] / self.nufft.ax.gamma - 0.5 * self.nufft.Kx) + 1, self.nufft.Ktx)
+                key = (ay + self.nufft.Kty * ax).astype(np.int64)
+                b = np.concatenate([[0], np.cumsum(P.blocks[l])])
+                idx = np.concatenate([b[i] + np.argsort(key[b[i]:b[i + 1]], kind='stable')
+                                      for i in range(len(P.blocks[l]))])
+            else:
+                idx = np.arange(len(P.u[l]))
+            self.order.append(idx)
+        self.u = [P.u[l][self.order[l]] for l in range(P.L)]
+        self.v = [P.v[l][self.order[l]] for l in range(P.L)]
+        self.y_host = [np.asarray(P.y[l])[self.order[l]] for l in range(P.L)]
+        self.G = [self.nufft.gmatrix(ctx, self.u[l], self.v[l], row_scale=np.full(len(self.u[l]), 1.0 / P.sigma_noise[l]))
                   for l in range(P.L)]
         self.D = DataOperator(ctx, self.plan, self.G, P.blocks)
-        self.y = torch.from_numpy(np.concatenate(P.y).astype(np.complex64)).cuda()
+        self.y = torch.from_numpy(np.concatenate(self.y_host).astype(np.complex64)).cuda()
         if precond:
-            pu = [uniform_weights(P.u[l], P.v[l], self.nufft.Kty, self.nufft.Ktx) for l in range(P.L)]
-            self.pU = torch.from_numpy(np.concatenate(pu).astype(np.float32)).cuda()
+            self.pU_host = [uniform_weights(self.u[l], self.v[l], self.nufft.Kty, self.nufft.Ktx) for l in range(P.L)]
+            self.pU = torch.from_numpy(np.concatenate(self.pU_host).astype(np.float32)).cuda()
         else:
+            self.pU_host = [np.ones(len(u)) for u in self.u]
             self.pU = None
         self.epsilon = np.concatenate([np.asarray(e, dtype=np.float64) for e in P.epsilon])
 
@@ -200,6 +220,9 @@ class PdfbEngine:
     def set_graph(self, on):
         self.ctx.check(lib().fhs_pdfb_set_graph(self.h, int(bool(on))), 'pdfb_set_graph')
 
+    def set_concurrent(self, on):
+        self.ctx.check(lib().fhs_pdfb_set_concurrent(self.h, int(bool(on))), 'pdfb_set_concurrent')
+
     def iterate(self, n=1):
         out = np.zeros(4 * n)
         self.ctx.check(lib().fhs_pdfb_iterate(self.h, int(n), out.ctypes.data_as(C.POINTER(C.c_double))),
@@ -238,7 +261,7 @@ def snr_db(X0, x):
 
 
 def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, graph=True,
-           max_total_iter=None, time_limit=None):
+           max_total_iter=None, time_limit=None, concurrent=True):
     """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'."""
     prm = dict(DEFAULT_PARAM)
     prm.update(param or {})
@@ -253,6 +276,7 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     xi = None if x_init is None else _cube(x_init)
     eng = PdfbEngine(meas, sara, lowrank, xi)
     eng.set_graph(graph)
+    eng.set_concurrent(concurrent)
     # operator norms and regularisation (imaging/imaging.m:728,825-860)
     son = prm.get('squared_operator_norm')
     if son is None:

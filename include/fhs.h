@@ -55,6 +55,11 @@ int fhs_ctx_set_timing(fhs_ctx* ctx, int enable);
 int fhs_ctx_timing_report(fhs_ctx* ctx, char* names_buf, int names_cap,
                           double* total_ms, long* counts, int cap, int* n_out);
 int fhs_ctx_timing_reset(fhs_ctx* ctx);
+/* Device memory for FFI callers without an allocator (MEX, ctypes). */
+int fhs_device_alloc(fhs_ctx* ctx, int64_t bytes, void** ptr);
+int fhs_device_free(fhs_ctx* ctx, void* ptr);
+int fhs_memcpy_h2d(fhs_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int fhs_memcpy_d2h(fhs_ctx* ctx, void* dst, const void* src, int64_t bytes);
 
 /* ------------------------------------------------ interpolation matrix -- */
 /* G from MATLAB-style CSR rows (rowptr[M+1], colidx[nnz], vals[2*nnz]
@@ -72,6 +77,11 @@ int fhs_gmat_create_csr_host(fhs_ctx* ctx, int64_t M, int64_t ncols,
 int fhs_gmat_create_window_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx,
                                 int Ky, int Kx, const int32_t* anchor,
                                 const float* coef, fhs_gmat** out);
+/* G straight from a MATLAB sparse matrix (CSC: mxGetJc / mxGetIr as
+ * int64, interleaved complex values of the R2018a API, host pointers). */
+int fhs_gmat_create_csc_host(fhs_ctx* ctx, int64_t M, int64_t ncols,
+                             const int64_t* colptr, const int64_t* rowidx,
+                             const float* vals, fhs_gmat** out);
 int fhs_gmat_destroy(fhs_gmat* g);
 int fhs_gmat_info(const fhs_gmat* g, int64_t* M, int64_t* ncols, int64_t* nnz);
 
@@ -106,6 +116,15 @@ int fhs_apply_scaled_fourier_adjoint(fhs_ctx* ctx, const fhs_nufft* p,
  * x, y, p: complex[n] (p may alias x). */
 int fhs_proj_l2ball(fhs_ctx* ctx, const void* x, double eps, const void* y,
                     int64_t n, void* p);
+
+/* [epsilon, t_block] = update_epsilon(epsilon, t, t_block, norm_res,
+ *   adapt_eps_tol_in, adapt_eps_tol_out, adapt_eps_steps,
+ *   adapt_eps_change_percentage)   src/update_epsilon.m:1-73
+ * Cells {L}{nblocks} flattened to n blocks (host arrays, updated in place);
+ * n_updated (may be NULL) counts the changes the reference prints. */
+int fhs_update_epsilon(double* epsilon, int t, int* t_block, const double* norm_res, int n,
+                       double adapt_eps_tol_in, double adapt_eps_tol_out, int adapt_eps_steps,
+                       double adapt_eps_change_percentage, int* n_updated);
 
 /* ------------------------------------------------------------- primal -- */
 /* xsol <- max(xsol - (g_a + g_b + g_c), 0); xhat = 2 xsol - xsol_prev;
@@ -259,6 +278,8 @@ int fhs_pdfb_set_params(fhs_pdfb* p, const double* thresh0, double beta1, float 
                         int flag_dimensionality_reduction);
 /* replay iterations from a captured hipGraph (default on) */
 int fhs_pdfb_set_graph(fhs_pdfb* p, int enable);
+/* run the prior steps on a second stream beside the data step (default on) */
+int fhs_pdfb_set_concurrent(fhs_pdfb* p, int enable);
 /* runs n_iter iterations; scalars: host double[4*n_iter] or NULL:
  * sum (x_prev - x)^2, sum x^2, sum_b norm_res_b^2, sum_b eps_b^2 */
 int fhs_pdfb_iterate(fhs_pdfb* p, int n_iter, double* scalars);

@@ -186,3 +186,83 @@ def sara_solver(problem, param, x_init=None, X0=None, max_iter_total=None):
         if X0 is not None:
             info['snr'].append(snr(X0.reshape(Ny, Nx, 1), xsol))
     return xsol[:, :, 0], info
+
+
+def facet_hypersara(problem, param, geom, x_init=None, X0=None, max_iter_total=None):
+    """PDFB of ``src/fhs/facetHyperSARA.m:701-889`` on one process: faceted
+    weighted nuclear norm (overlapping facets ``geom`` with apodisation
+    windows, per-facet beta0_q = gamma0_q / sigma0, ``:598-601``) and the
+    SARA l21 term whose facet coefficient sets tile the global transform, so
+    the global l21 step is the union of the facet steps
+    (``src/fhs/fhs_update_dual_sparsity.m:67-93``).  Stopping rule with the
+    data-fidelity check (``:877-881``).  param as ``hypersara`` with
+    ``sig_bar`` per facet."""
+    from .facets import faceted_lowrank_step, facet_singular_values
+    P = problem
+    Ny, Nx, L = P.Ny, P.Nx, P.L
+    Q = len(geom['oy'])
+    sara = SaraDictionary(Ny, Nx)
+    xsol = np.zeros((Ny, Nx, L)) if x_init is None else np.array(x_init, dtype=np.float64)
+    g = np.zeros_like(xsol)
+    sig_bar = np.asarray(param['sig_bar'], dtype=np.float64).reshape(Q)
+    w0 = np.ones((Q, L))
+    w1 = np.ones(sara.s)
+    if np.any(xsol):
+        ub = sig_bar[:, None] * param['alpha']
+        w0 = ub / (ub + facet_singular_values(xsol, geom))
+        _, w1 = hs_initial_weights(xsol, sara, param['alpha'], param['sig'], 1.0)
+    v0s = [np.zeros((geom['py'][q] * geom['px'][q], L)) for q in range(Q)]
+    v1 = np.zeros((sara.s, L))
+    v2 = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[l]] for l in range(L)]
+    proj = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[l]] for l in range(L)]
+    Fx_old = np.stack([P.A[l](xsol[:, :, l]) for l in range(L)], axis=1)
+    tau = 0.99 / 3
+    sigma00, sigma11 = tau, tau
+    sigma22 = tau / np.asarray(param['nu2'], dtype=np.float64)
+    beta0 = np.asarray(param['gamma0'], dtype=np.float64).reshape(Q)
+    beta1 = param['gamma']
+    max_iter = (param['reweighting_max_iter'] + 1) * param['pdfb_max_iter']
+    if max_iter_total is not None:
+        max_iter = min(max_iter, max_iter_total)
+    info = {'rel_val': [], 'snr': [], 'norm_res': [], 'norm_eps': [], 'reweights': []}
+    rw_count, last_step = 0, 0
+    xlast = xsol.copy()
+    t = 0
+    for t in range(1, max_iter + 1):
+        prev = xsol
+        xsol, xhat, _, _ = update_primal(xsol, g)
+        v0s, g0 = faceted_lowrank_step(v0s, xhat, geom, beta0[:, None] * w0, 1.0, sigma00)
+        r1 = v1 + np.stack([sara.psit(xhat[:, :, l]) for l in range(L)], axis=1)
+        v1 = soft_l21(r1, beta1 * w1)
+        g1 = np.stack([sigma11 * sara.psi(v1[:, l]) for l in range(L)], axis=2)
+        v2, g2, Fx_old, proj, norm_res, gnr, neps = update_dual_data_fidelity(
+            v2, P.y, xsol, Fx_old, proj, P.A, P.At, P.G, P.W, P.pU, P.epsilon,
+            param['elipse_proj_max_iter'], param['elipse_proj_min_iter'], param['elipse_proj_eps'], sigma22)
+        g = g0 + g1 + g2
+        nx = np.linalg.norm(xsol.ravel())
+        rel_val = np.linalg.norm((prev - xsol).ravel()) / nx if nx > 0 else np.nan
+        info['rel_val'].append(rel_val)
+        info['norm_res'].append(np.sqrt(gnr))
+        info['norm_eps'].append(np.sqrt(neps))
+        if X0 is not None:
+            info['snr'].append(snr(X0, xsol))
+        k = t - last_step
+        conv = (k >= param['pdfb_min_iter']) and (
+            k >= param['pdfb_max_iter'] or
+            (rel_val <= param['pdfb_rel_var'] and np.sqrt(gnr) <= param['pdfb_fidelity_tolerance'] * np.sqrt(neps))
+            or rel_val <= param['pdfb_rel_var_low'])
+        if conv:
+            rel_rw = np.linalg.norm((xlast - xsol).ravel()) / np.linalg.norm(xlast.ravel()) \
+                if np.any(xlast) else np.inf
+            xlast = xsol.copy()
+            if (rw_count >= param['reweighting_min_iter'] and
+                    (rw_count >= param['reweighting_max_iter'] or rel_rw <= param['reweighting_rel_var'])):
+                break
+            ub = sig_bar[:, None] * param['alpha']
+            w0 = ub / (ub + facet_singular_values(xsol, geom))
+            _, w1 = hs_initial_weights(xsol, sara, param['alpha'], param['sig'], 1.0)
+            rw_count += 1
+            last_step = t
+            info['reweights'].append(t)
+    info['iterations'] = t
+    return xsol, info
