@@ -43,16 +43,20 @@ def parse():
 
 
 def algorithmic_bytes(meas, sara_s):
-    """Per-launch algorithmic HBM bytes of the kernels (formulas of the
-    design notes: complex = 8 B, real FP32 = 4 B)."""
+    """Per-launch algorithmic HBM bytes of the kernels for the layouts in use
+    (complex = 8 B, real FP32 = 4 B; every operand read or written once).
+    G is the separable min-max layout: Ky + Kx coefficients and an anchor
+    per row, 4-byte packed CSC entries for the adjoint."""
     L, No = meas.L, meas.nufft.No
     M = [g.M for g in meas.G]
     nnz = [g.nnz for g in meas.G]
     N = meas.Ny * meas.Nx
-    g_bytes = sum(8 * k + 4 * m + 8 * No + 8 * m for k, m in zip(nnz, M))
+    row = 8 * (meas.nufft.Ky + meas.nufft.Kx) + 4
+    fwd = sum(row * m + 8 * No + 24 * m for m in M)                      # coefs, Fx, gx_old r/w, r2
+    adj = sum(4 * k + 12 * No + (row - 4 + 8) * m for k, m in zip(nnz, M))  # CSC, colptr+out, coefs+v2
     return {
-        'data_adjoint': g_bytes,
-        'data_forward': g_bytes,
+        'data_adjoint': adj,
+        'data_forward': fwd,
         'fft_fwd_y': 4 * N * L + 8 * (No // 2) * L,          # read image, write half-grid
         'fft_fwd_x': 8 * (No // 2) * L + 8 * No * L,
         'fft_adj_x': 8 * No * L + 8 * (No // 2) * L,
@@ -103,11 +107,14 @@ def main():
     iters_per_s = args.steps / elapsed
 
     # ---- live per-kernel timing (HIP events on the engine stream) ---------
+    # kernels in issue order on one stream, so each duration is its own
+    eng.set_concurrent(False)
     ctx.timing_reset()
     ctx.set_timing(True)
     n_timed = 10
     eng.iterate(n_timed)
     ctx.set_timing(False)
+    eng.set_concurrent(True)
     rep = ctx.timing_report()
     ctx.timing_reset()
     alg = algorithmic_bytes(meas, eng.sara.s)
@@ -187,7 +194,9 @@ def main():
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1),
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
                      'traffic': None, 'avg_launch_ms': round(avg_ms, 5),
-                     'algorithmic_bytes_per_launch': int(alg[dominant])},
+                     'algorithmic_bytes_per_launch': int(alg[dominant]),
+                     'survey_formula_bytes_per_launch': int(sum(8 * g.nnz + 12 * g.M + 8 * meas.nufft.No
+                                                                for g in meas.G))},
         'kernels': kernels,
         'solve_plain_fp32': solve,
         'cpu_baseline': cpu,

@@ -46,15 +46,34 @@ namespace fhs {
 namespace {
 
 constexpr int FB_THREADS = 1024;
+constexpr int NXCD = 8;   // MI355X: 8 XCDs, each with its own L2
+
+// Workgroups of a launch are dispatched to the XCDs round-robin (linear id
+// mod 8).  The per-channel kernels below are launched as a 1-D grid of
+// NXCD * ceil(L / NXCD) * units workgroups and remapped so that XCD x only
+// runs channels x, x + 8, ...: one channel's v2 / coefficients / grid
+// region is then shared by one L2 instead of being pulled into all eight.
+__device__ __forceinline__ bool xcd_map(int L, int units, int& chan, int& unit) {
+  const int b = blockIdx.x;
+  const int xcd = b % NXCD, s = b / NXCD;
+  chan = xcd + NXCD * (s / units);
+  unit = s % units;
+  return chan < L;
+}
+
+inline unsigned xcd_grid(int L, int units) {
+  return (unsigned)(NXCD * ((L + NXCD - 1) / NXCD) * units);
+}
 
 __global__ __launch_bounds__(256) void k_data_forward(const GDev* __restrict__ gd,
                                                       const int64_t* __restrict__ row_off,
                                                       const float2* __restrict__ Fx, int64_t No,
                                                       float2* __restrict__ gx_old,
-                                                      float2* __restrict__ r2) {
-  const int l = blockIdx.y;
+                                                      float2* __restrict__ r2, int L, int units) {
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = u * blockDim.x + threadIdx.x;
   if (m >= g.M) return;
   const float2 gx = g_row_dot(g, m, Fx + (size_t)l * No);
   const int64_t i = row_off[l] + m;
@@ -70,10 +89,11 @@ __global__ __launch_bounds__(256) void k_data_forward_dr(const GDev* __restrict_
                                                          const float2* __restrict__ Fx, int64_t No,
                                                          const float* __restrict__ lam,
                                                          float2* __restrict__ gx_old,
-                                                         float2* __restrict__ r2) {
-  const int l = blockIdx.y;
+                                                         float2* __restrict__ r2, int L, int units) {
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = u * blockDim.x + threadIdx.x;
   if (m >= g.M) return;
   const float2* F = Fx + (size_t)l * No;
   float2 gx = cadd(g_row_dot(g, m, F), g_col_dot(g, m, [&](int r) { return F[r]; }));
@@ -197,12 +217,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint(const GDev* __rest
                                                               const int64_t* __restrict__ row_off,
                                                               const float2* __restrict__ v2,
                                                               const float* __restrict__ lam, int dr,
-                                                              int64_t No, float2* __restrict__ g2) {
+                                                              int64_t No, float2* __restrict__ g2, int L,
+                                                              int units) {
   __shared__ float2 prod[ADJ_LDS];
-  const int l = blockIdx.y;
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
   float2* out = g2 + (size_t)l * No;
-  const int u = blockIdx.x;
   if (u >= g.nunits) {
     // zero the grid tail beyond the matrix columns (only when ncols < No)
     const int64_t extra_units = (No - g.ncols + ADJ_COLS - 1) / ADJ_COLS;
@@ -237,10 +258,11 @@ __global__ __launch_bounds__(256) void k_data_apply(const GDev* __restrict__ gd,
                                                     const int64_t* __restrict__ row_off,
                                                     const float2* __restrict__ Fx, int64_t No,
                                                     const float* __restrict__ lam,
-                                                    float2* __restrict__ y) {
-  const int l = blockIdx.y;
+                                                    float2* __restrict__ y, int L, int units) {
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = u * blockDim.x + threadIdx.x;
   if (m >= g.M) return;
   const int64_t i = row_off[l] + m;
   float2 v = g_row_dot(g, m, Fx + (size_t)l * No);
@@ -280,11 +302,13 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
   nufft_forward(ctx, D->A, x, L, D->Fx);
   {
     Scope sc(ctx, "data_forward");
-    dim3 grid(cdiv(D->maxM, 256), L);
+    const int units = (int)cdiv(D->maxM, 256);
+    const unsigned grid = xcd_grid(L, units);
     if (dr)
-      k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2);
+      k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2, L,
+                                                       units);
     else
-      k_data_forward<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2);
+      k_data_forward<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, L, units);
     check_launch();
   }
   {
@@ -299,8 +323,8 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
   }
   {
     Scope sc(ctx, "data_adjoint");
-    dim3 grid(D->adj_grid, L);
-    k_data_adjoint<<<grid, ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr, D->No, D->g2);
+    k_data_adjoint<<<xcd_grid(L, D->adj_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr,
+                                                                              D->No, D->g2, L, D->adj_grid);
     check_launch();
   }
   nufft_adjoint(ctx, D->A, D->g2, L, D->d_sigma22, Ftx);
@@ -322,11 +346,13 @@ void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int
   // gx_old = G A(x): run the forward with a zero previous state, keep gx
   nufft_forward(ctx, D->A, x, D->L, D->Fx);
   FHS_HIP(hipMemsetAsync(gx_old, 0, D->Mtot * sizeof(float2), ctx->stream));
-  dim3 grid(cdiv(D->maxM, 256), D->L);
+  const int units = (int)cdiv(D->maxM, 256);
+  const unsigned grid = xcd_grid(D->L, units);
   if (dr)
-    k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2);
+    k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2,
+                                                     D->L, units);
   else
-    k_data_forward<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2);
+    k_data_forward<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L, units);
   check_launch();
 }
 
@@ -334,8 +360,9 @@ void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int
 void data_forward(fhs_ctx* ctx, fhs_data* D, const float* x, const float* lam, float2* y) {
   nufft_forward(ctx, D->A, x, D->L, D->Fx);
   Scope sc(ctx, "data_forward");
-  k_data_apply<<<dim3(cdiv(D->maxM, 256), D->L), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No,
-                                                                        lam, y);
+  const int units = (int)cdiv(D->maxM, 256);
+  k_data_apply<<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L,
+                                                               units);
   check_launch();
 }
 
@@ -344,8 +371,8 @@ void data_adjoint(fhs_ctx* ctx, fhs_data* D, const float2* v, const float* lam, 
                   float* out) {
   {
     Scope sc(ctx, "data_adjoint");
-    dim3 grid(D->adj_grid, D->L);
-    k_data_adjoint<<<grid, ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v, lam, 0, D->No, D->g2);
+    k_data_adjoint<<<xcd_grid(D->L, D->adj_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v, lam, 0,
+                                                                                 D->No, D->g2, D->L, D->adj_grid);
     check_launch();
   }
   nufft_adjoint(ctx, D->A, D->g2, D->L, scale_dev, out);

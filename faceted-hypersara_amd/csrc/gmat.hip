@@ -217,6 +217,78 @@ int fhs_gmat_create_window_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, int K
   });
 }
 
+int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, int Ky, int Kx,
+                                   const int32_t* anchor, const float* cy, const float* cx,
+                                   fhs_gmat** out) {
+  if (!ctx || !out) return FHS_ERR_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    FHS_CHECK(M > 0 && M < (1ll << 26), "gmat_create_separable: M must be in [1, 2^26)");
+    FHS_CHECK(Ky >= 1 && Kx >= 1 && Ky <= SEP_KMAX && Kx <= SEP_KMAX && Ky * Kx <= 64 && Ky <= Kty &&
+                  Kx <= Ktx,
+              "gmat_create_separable: 1 <= Ky, Kx <= 8");
+    const int64_t ncols = (int64_t)Kty * Ktx;
+    FHS_CHECK(ncols < (1ll << 31), "gmat_create_separable: grid too large");
+    const int K = Ky * Kx;
+    const int64_t nnz = M * K;
+    FHS_CHECK(nnz < (1ll << 31), "gmat_create_separable: nnz out of range");
+    std::vector<float2> sep((size_t)(Ky + Kx) * M);
+    std::vector<int> anc(anchor, anchor + M);
+    std::vector<int> colptr(ncols + 1, 0);
+    for (int64_t m = 0; m < M; ++m) {
+      const int a = anchor[m];
+      FHS_CHECK(a >= 0 && a < ncols, "gmat_create_separable: anchor out of range");
+      for (int jy = 0; jy < Ky; ++jy)
+        sep[(size_t)jy * M + m] = make_float2(cy[2 * (m * Ky + jy)], cy[2 * (m * Ky + jy) + 1]);
+      for (int jx = 0; jx < Kx; ++jx)
+        sep[(size_t)(Ky + jx) * M + m] = make_float2(cx[2 * (m * Kx + jx)], cx[2 * (m * Kx + jx) + 1]);
+      const int ky0 = a % Kty, kx0 = a / Kty;
+      for (int jx = 0; jx < Kx; ++jx)
+        for (int jy = 0; jy < Ky; ++jy) colptr[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx) + 1]++;
+    }
+    for (int64_t k = 0; k < ncols; ++k) colptr[k + 1] += colptr[k];
+    std::vector<int> pos(colptr.begin(), colptr.end() - 1);
+    std::vector<unsigned> pk(nnz);
+    for (int64_t m = 0; m < M; ++m) {   // rows ascending inside every column
+      const int a = anchor[m];
+      const int ky0 = a % Kty, kx0 = a / Kty;
+      for (int jx = 0; jx < Kx; ++jx)
+        for (int jy = 0; jy < Ky; ++jy) {
+          const int col = (ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx);
+          pk[pos[col]++] = ((unsigned)m << 6) | (unsigned)(jx * Ky + jy);
+        }
+    }
+    const std::vector<int> units = adjoint_units(colptr);
+    FHS_HIP(hipSetDevice(ctx->device));
+    fhs_gmat* g = new fhs_gmat();
+    g->device = ctx->device;
+    g->M = M;
+    g->ncols = ncols;
+    g->nnz = nnz;
+    g->mem[0] = upload(anc);
+    g->mem[1] = upload(sep);
+    g->mem[2] = upload(colptr);
+    g->mem[3] = upload(pk);
+    g->mem[6] = upload(units);
+    GDev& d = g->d;
+    d.layout = G_SEP;
+    d.M = (int)M;
+    d.K = K;
+    d.Kty = Kty;
+    d.Ktx = Ktx;
+    d.Ky = Ky;
+    d.Kx = Kx;
+    d.ncols = (int)ncols;
+    d.anchor = (const int*)g->mem[0];
+    d.vals = (const float2*)g->mem[1];
+    d.colptr = (const int*)g->mem[2];
+    d.csc_pk = (const unsigned*)g->mem[3];
+    d.ucol = (const int*)g->mem[6];
+    d.nunits = (int)units.size() - 1;
+    *out = g;
+  });
+}
+
 int fhs_gmat_destroy(fhs_gmat* g) {
   if (!g) return FHS_OK;
   (void)hipSetDevice(g->device);

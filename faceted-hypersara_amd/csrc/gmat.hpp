@@ -8,6 +8,11 @@
 //   WINDOW : one anchor per row; tap j = jx*Ky + jy reads grid point
 //            ((ky0 + jy) mod Kty, (kx0 + jx) mod Ktx) - the dense min-max
 //            window, no column indices (8 B per nonzero instead of 12).
+//   SEP    : the min-max interpolator of a 2-D NUFFT is the Kronecker
+//            product of two 1-D interpolators, G[m, (jy, jx)] =
+//            cy[m][jy] cx[m][jx]: a row stores Ky + Kx complex values
+//            (112 B for 7 x 7 instead of 392 B) plus its anchor.  The CSC of
+//            the adjoint then stores only (row << 6 | tap) per nonzero (4 B).
 // Adjoint (gridding, Fx = G' y) is computed per grid point from a CSC copy
 // (colptr / rows / conj-ready values): each output is owned by one thread,
 // no atomics, deterministic summation order.
@@ -17,7 +22,8 @@
 
 namespace fhs {
 
-enum GLayout { G_ELL = 0, G_WINDOW = 1 };
+enum GLayout { G_ELL = 0, G_WINDOW = 1, G_SEP = 2 };
+constexpr int SEP_KMAX = 8;   // max taps per axis of the separable layout
 
 struct GDev {
   int layout;
@@ -32,7 +38,21 @@ struct GDev {
   const float2* csc_vals;
   const int* ucol;      // adjoint work units: unit u = columns [ucol[u], ucol[u+1])
   int nunits;
+  const unsigned* csc_pk;  // SEP: (row << 6) | (jx*Ky + jy) per nonzero
 };
+
+// value and row of CSC entry e
+__device__ __forceinline__ void g_csc_entry(const GDev& g, int e, float2& v, int& r) {
+  if (g.layout == G_SEP) {
+    const unsigned pk = g.csc_pk[e];
+    r = (int)(pk >> 6);
+    const int tap = (int)(pk & 63u), jx = tap / g.Ky, jy = tap - jx * g.Ky;
+    v = cmul(g.vals[(size_t)jy * g.M + r], g.vals[(size_t)(g.Ky + jx) * g.M + r]);
+  } else {
+    v = g.csc_vals[e];
+    r = g.csc_rows[e];
+  }
+}
 
 // Adjoint work units: contiguous column ranges holding about ADJ_ENTRIES
 // nonzeros (and at most ADJ_COLS columns), so every workgroup streams the
@@ -53,6 +73,32 @@ __device__ __forceinline__ float2 g_row_apply(const GDev& g, int m, F&& f) {
       const float2 x = f(c);
       acc.x += v.x * x.x - v.y * x.y;
       acc.y += v.x * x.y + v.y * x.x;
+    }
+  } else if (g.layout == G_SEP) {
+    const int a = g.anchor[m];
+    const int ky0 = a % g.Kty, kx0 = a / g.Kty;
+    float2 cy[SEP_KMAX];
+#pragma unroll
+    for (int jy = 0; jy < SEP_KMAX; ++jy)
+      cy[jy] = jy < g.Ky ? g.vals[(size_t)jy * g.M + m] : make_float2(0.f, 0.f);
+    for (int jx = 0; jx < g.Kx; ++jx) {
+      int kx = kx0 + jx;
+      if (kx >= g.Ktx) kx -= g.Ktx;
+      const int colbase = kx * g.Kty;
+      float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int jy = 0; jy < SEP_KMAX; ++jy) {
+        if (jy < g.Ky) {
+          int ky = ky0 + jy;
+          if (ky >= g.Kty) ky -= g.Kty;
+          const float2 x = f(colbase + ky);
+          s.x += cy[jy].x * x.x - cy[jy].y * x.y;
+          s.y += cy[jy].x * x.y + cy[jy].y * x.x;
+        }
+      }
+      const float2 c = g.vals[(size_t)(g.Ky + jx) * g.M + m];
+      acc.x += c.x * s.x - c.y * s.y;
+      acc.y += c.x * s.y + c.y * s.x;
     }
   } else {
     const int a = g.anchor[m];
@@ -85,8 +131,10 @@ __device__ __forceinline__ float2 g_col_dot(const GDev& g, int k, W&& w) {
   float2 acc = make_float2(0.f, 0.f);
   const int e0 = g.colptr[k], e1 = g.colptr[k + 1];
   for (int e = e0; e < e1; ++e) {
-    const float2 v = g.csc_vals[e];
-    const float2 y = w(g.csc_rows[e]);
+    float2 v;
+    int r;
+    g_csc_entry(g, e, v, r);
+    const float2 y = w(r);
     acc.x += v.x * y.x + v.y * y.y;
     acc.y += v.x * y.y - v.y * y.x;
   }
@@ -105,8 +153,10 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
   const int e0 = g.colptr[c0], n = g.colptr[c1] - e0;
   if (n <= ADJ_LDS) {
     for (int e = threadIdx.x; e < n; e += ADJ_THREADS) {
-      const float2 v = g.csc_vals[e0 + e];
-      const float2 y = w(g.csc_rows[e0 + e]);
+      float2 v;
+      int r;
+      g_csc_entry(g, e0 + e, v, r);
+      const float2 y = w(r);
       prod[e] = make_float2(v.x * y.x + v.y * y.y, v.x * y.y - v.y * y.x);
     }
     __syncthreads();

@@ -84,10 +84,28 @@ __global__ __launch_bounds__(GRAM_THREADS) void k_gram(FacetDev F, int L, const 
   partial[((int64_t)q * nchunk + chunk) * NUC_L * NUC_L + threadIdx.x] = acc;
 }
 
+// FP64 sum of the per-chunk partial Grams: one workgroup per facet, thread
+// e owns Gram entry e and keeps 8 independent accumulators so the chunk
+// loads stay in flight (a single dependent chain is latency-bound).
+__global__ __launch_bounds__(NUC_L * NUC_L) void k_gram_reduce(const double* __restrict__ partial, int nchunk,
+                                                               const int* __restrict__ nchunk_q,
+                                                               double* __restrict__ gram) {
+  const int q = blockIdx.x, e = threadIdx.x;
+  const int nc = nchunk_q[q];
+  const double* pq = partial + (int64_t)q * nchunk * NUC_L * NUC_L + e;
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int k = 0;
+  for (; k + 8 <= nc; k += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] += pq[(int64_t)(k + j) * NUC_L * NUC_L];
+  }
+  for (; k < nc; ++k) a[0] += pq[(int64_t)k * NUC_L * NUC_L];
+  gram[(int64_t)q * NUC_L * NUC_L + e] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 // One wavefront per facet.  mode 0: dual step (writes M); mode 1: singular
 // values only.  Vw: [Q][16][16] FP64 warm-start eigenvectors (in/out).
-__global__ __launch_bounds__(64) void k_eig(int L, const double* __restrict__ partial, int nchunk,
-                                            const int* __restrict__ nchunk_q, double* __restrict__ Vw,
+__global__ __launch_bounds__(64) void k_eig(int L, const double* __restrict__ gram, double* __restrict__ Vw,
                                             const double* __restrict__ thresh, int mode,
                                             float* __restrict__ Mout, double* __restrict__ sv) {
   __shared__ double A[NUC_L][NUC_L + 1];
@@ -99,13 +117,9 @@ __global__ __launch_bounds__(64) void k_eig(int L, const double* __restrict__ pa
   __shared__ int order[NUC_L];
   __shared__ int done;
   const int q = blockIdx.x, t = threadIdx.x;
-  const int nc = nchunk_q[q];
-  // 1. Gram in FP64
+  // 1. Gram (FP64, reduced by k_gram_reduce)
   for (int e = t; e < NUC_L * NUC_L; e += 64) {
-    double s = 0.0;
-    const double* pq = partial + (int64_t)q * nchunk * NUC_L * NUC_L + e;
-    for (int k = 0; k < nc; ++k) s += pq[(int64_t)k * NUC_L * NUC_L];
-    T[e / NUC_L][e % NUC_L] = s;
+    T[e / NUC_L][e % NUC_L] = gram[(int64_t)q * NUC_L * NUC_L + e];
     V[e / NUC_L][e % NUC_L] = Vw[(int64_t)q * NUC_L * NUC_L + e];
   }
   __syncthreads();
@@ -338,6 +352,7 @@ struct fhs_lowrank {
   double* d_sv = nullptr;              // [Q][16]
   float* d_M = nullptr;                // [Q][16][16]
   double* d_partial = nullptr;         // [Q][nchunk][256]
+  double* d_gram = nullptr;            // [Q][256]
 };
 
 namespace fhs {
@@ -351,8 +366,10 @@ static void lowrank_run(fhs_ctx* ctx, fhs_lowrank* R, const float* v0, const flo
   }
   {
     Scope sc(ctx, "nuclear_eig");
-    k_eig<<<R->Q, 64, 0, ctx->stream>>>(R->L, R->d_partial, R->nchunk, R->d_geom + 4 * R->Q, R->d_V,
-                                        R->d_thresh, mode, R->d_M, R->d_sv);
+    k_gram_reduce<<<R->Q, NUC_L * NUC_L, 0, ctx->stream>>>(R->d_partial, R->nchunk, R->d_geom + 4 * R->Q,
+                                                           R->d_gram);
+    check_launch();
+    k_eig<<<R->Q, 64, 0, ctx->stream>>>(R->L, R->d_gram, R->d_V, R->d_thresh, mode, R->d_M, R->d_sv);
     check_launch();
   }
 }
@@ -479,6 +496,7 @@ int fhs_lowrank_create_host(fhs_ctx* ctx, int Ny, int Nx, int L, int Qy, int Qx,
     FHS_HIP(hipMalloc(&R->d_sv, (size_t)Q * NUC_L * sizeof(double)));
     FHS_HIP(hipMalloc(&R->d_M, (size_t)Q * NUC_L * NUC_L * sizeof(float)));
     FHS_HIP(hipMalloc(&R->d_partial, (size_t)Q * R->nchunk * NUC_L * NUC_L * sizeof(double)));
+    FHS_HIP(hipMalloc(&R->d_gram, (size_t)Q * NUC_L * NUC_L * sizeof(double)));
     FacetDev& F = R->F;
     F.Q = Q;
     F.Ny = Ny;
@@ -498,7 +516,7 @@ int fhs_lowrank_destroy(fhs_lowrank* R) {
   if (!R) return FHS_OK;
   (void)hipSetDevice(R->device);
   void* ps[] = {R->d_geom, R->d_off, R->d_win, R->d_rowf, R->d_colf, R->d_V,
-                R->d_thresh, R->d_sv, R->d_M, R->d_partial};
+                R->d_thresh, R->d_sv, R->d_M, R->d_partial, R->d_gram};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete R;

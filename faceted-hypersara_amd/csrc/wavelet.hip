@@ -11,9 +11,8 @@
 //
 // Kernels work on T x T output tiles: the input window (plus filter halo) is
 // staged in LDS, filtered along x then y (separable, 2 lf FMAs per output).
-// Level 1 loads the image window once per tile and loops over the bases
-// (their windows nest inside the longest filter's); levels >= 2 run one
-// workgroup per (tile, basis, channel).  Synthesis mirrors the analysis,
+// Analysis runs one workgroup per (tile, basis, channel) at every level
+// (level 1 re-reads the image window per basis from L2).  Synthesis mirrors the analysis,
 // and its finest level sums all bases and the Dirac part in registers so the
 // image is written once.
 #include <algorithm>
@@ -130,44 +129,8 @@ __device__ void load_window(float* win, const float* img, int ny, int nx, int wy
 
 constexpr int WIN_MAX = 2 * WT + WMAX_LF - 2;   // 78
 
-__global__ __launch_bounds__(WTHREADS) void k_an_l1(SaraGeom G, const float* __restrict__ x,
+__global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const float* __restrict__ x,
                                                     float* __restrict__ v1, float* __restrict__ scr) {
-  __shared__ float win[WIN_MAX * WIN_MAX];
-  __shared__ float Lx[WT * WIN_MAX];
-  __shared__ float Hx[WT * WIN_MAX];
-  __shared__ float flo[WMAX_LF], fhi[WMAX_LF];
-  const int l = blockIdx.z;
-  const int ky0 = blockIdx.x * WT, kx0 = blockIdx.y * WT;
-  int lfmax = 2;
-  for (int b = 0; b < G.nb; ++b) lfmax = max(lfmax, G.lf[b]);
-  const int WH = 2 * WT + lfmax - 2;
-  const float* img = x + (int64_t)l * G.Ny * G.Nx;
-  load_window(win, img, G.Ny, G.Nx, 2 * ky0 + 2 - lfmax, 2 * kx0 + 2 - lfmax, WH, WH, G.norm);
-  float* v1l = v1 + (int64_t)l * G.s;
-  for (int b = 0; b < G.nb; ++b) {
-    const int lf = G.lf[b];
-    if (threadIdx.x < lf) {
-      flo[threadIdx.x] = G.lo[b][threadIdx.x];
-      fhi[threadIdx.x] = G.hi[b][threadIdx.x];
-    }
-    __syncthreads();
-    if (ky0 < G.ny[b][1] && kx0 < G.nx[b][1]) {
-      AnOut out;
-      out.v1 = v1l;
-      out.last = G.nlevel == 1;
-      out.approx = scr + (int64_t)l * G.scr_per_chan;
-      out.a_off = out.last ? G.boff[b] + G.aoff[b] : G.scr[b][1];
-      out.h_off = G.boff[b] + G.doff[b][1];
-      out.ny = G.ny[b][1];
-      out.nx = G.nx[b][1];
-      an_tile(win, WH, lfmax - lf, lf, flo, fhi, ky0, kx0, Lx, Hx, out);
-    }
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, float* __restrict__ v1,
-                                                    float* __restrict__ scr) {
   __shared__ float win[WIN_MAX * WIN_MAX];
   __shared__ float Lx[WT * WIN_MAX];
   __shared__ float Hx[WT * WIN_MAX];
@@ -182,8 +145,12 @@ __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, float* __
   }
   const int WH = 2 * WT + lf - 2;
   float* sc = scr + (int64_t)l * G.scr_per_chan;
-  load_window(win, sc + G.scr[b][k - 1], G.ny[b][k - 1], G.nx[b][k - 1], 2 * ky0 + 2 - lf,
-              2 * kx0 + 2 - lf, WH, WH, 1.f);
+  if (k == 1)   // level 1 reads the image, scaled by 1/sqrt(P)
+    load_window(win, x + (int64_t)l * G.Ny * G.Nx, G.Ny, G.Nx, 2 * ky0 + 2 - lf, 2 * kx0 + 2 - lf, WH, WH,
+                G.norm);
+  else
+    load_window(win, sc + G.scr[b][k - 1], G.ny[b][k - 1], G.nx[b][k - 1], 2 * ky0 + 2 - lf,
+                2 * kx0 + 2 - lf, WH, WH, 1.f);
   AnOut out;
   out.v1 = v1 + (int64_t)l * G.s;
   out.last = k == G.nlevel;
@@ -426,15 +393,15 @@ void sara_analysis_wavelets(fhs_ctx* ctx, const fhs_sara* S, const float* x, int
     mx = std::max(mx, G.nx[b][1]);
   }
   Scope sc(ctx, "wavelet_analysis");
-  k_an_l1<<<dim3(cdiv(my, WT), cdiv(mx, WT), L), WTHREADS, 0, ctx->stream>>>(G, x, v1, scr);
-  check_launch();
-  for (int k = 2; k <= G.nlevel; ++k) {
+  (void)my;
+  (void)mx;
+  for (int k = 1; k <= G.nlevel; ++k) {
     int ny = 0, nx = 0;
     for (int b = 0; b < G.nb; ++b) {
       ny = std::max(ny, G.ny[b][k]);
       nx = std::max(nx, G.nx[b][k]);
     }
-    k_an_lk<<<dim3(cdiv(ny, WT), cdiv(nx, WT), G.nb * L), WTHREADS, 0, ctx->stream>>>(G, k, v1, scr);
+    k_an_lk<<<dim3(cdiv(ny, WT), cdiv(nx, WT), G.nb * L), WTHREADS, 0, ctx->stream>>>(G, k, x, v1, scr);
     check_launch();
   }
 }

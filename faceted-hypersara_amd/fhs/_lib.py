@@ -34,6 +34,8 @@ PROTOTYPES = {
     'fhs_gmat_create_csr_host': [c_void_p, c_i64, c_i64, c_i64_p, c_i64_p, c_float_p, C.POINTER(c_void_p)],
     'fhs_gmat_create_window_host': [c_void_p, c_i64, c_int, c_int, c_int, c_int, c_i32_p, c_float_p,
                                     C.POINTER(c_void_p)],
+    'fhs_gmat_create_separable_host': [c_void_p, c_i64, c_int, c_int, c_int, c_int, c_i32_p, c_float_p,
+                                       c_float_p, C.POINTER(c_void_p)],
     'fhs_gmat_create_csc_host': [c_void_p, c_i64, c_i64, c_i64_p, c_i64_p, c_float_p, C.POINTER(c_void_p)],
     'fhs_gmat_destroy': [c_void_p],
     'fhs_device_alloc': [c_void_p, c_i64, C.POINTER(c_void_p)],

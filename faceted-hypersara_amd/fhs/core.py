@@ -122,6 +122,21 @@ class GMatrix:
             coef.ctypes.data_as(C.POINTER(C.c_float)), C.byref(h)), 'gmat_create_window')
         return cls(ctx, h, M, Kty * Ktx, M * Ky * Kx)
 
+    @classmethod
+    def from_separable(cls, ctx, anchor, cy, cx, Kty, Ktx):
+        """Kronecker rows G[m, (jy, jx)] = cy[m, jy] cx[m, jx] (cy (M, Ky), cx (M, Kx))."""
+        anchor = np.ascontiguousarray(anchor, dtype=np.int32)
+        M, Ky = cy.shape
+        Kx = cx.shape[1]
+        cyf = np.ascontiguousarray(np.asarray(cy, dtype=np.complex64).view(np.float32))
+        cxf = np.ascontiguousarray(np.asarray(cx, dtype=np.complex64).view(np.float32))
+        h = C.c_void_p()
+        ctx.check(lib().fhs_gmat_create_separable_host(
+            ctx.h, M, Kty, Ktx, Ky, Kx, anchor.ctypes.data_as(C.POINTER(C.c_int32)),
+            cyf.ctypes.data_as(C.POINTER(C.c_float)), cxf.ctypes.data_as(C.POINTER(C.c_float)), C.byref(h)),
+            'gmat_create_separable')
+        return cls(ctx, h, M, Kty * Ktx, M * Ky * Kx)
+
     def __del__(self):
         h = getattr(self, 'h', None)
         if h is not None:

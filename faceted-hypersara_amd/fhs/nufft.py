@@ -84,10 +84,22 @@ class Nufft2D:
             coef = coef * np.asarray(row_scale)[:, None]
         return anchor, coef
 
+    def separable(self, u, v, row_scale=None):
+        """anchor (M,), cy (M, Ky), cx (M, Kx): G(m, (jy, jx)) = cy[m, jy] cx[m, jx];
+        row_scale folded into cx."""
+        ay, cy = self.ay.coefficients(v)
+        ax, cx = self.ax.coefficients(u)
+        if row_scale is not None:
+            cx = cx * np.asarray(row_scale)[:, None]
+        return ay + self.Kty * ax, cy, cx
+
     def plan(self, ctx):
         return NufftPlan(ctx, self.Ny, self.Nx, self.Kty, self.Ktx, self.ay.s, self.ax.s)
 
-    def gmatrix(self, ctx, u, v, row_scale=None):
+    def gmatrix(self, ctx, u, v, row_scale=None, layout='separable'):
+        if layout == 'separable':
+            anchor, cy, cx = self.separable(u, v, row_scale)
+            return GMatrix.from_separable(ctx, anchor, cy, cx, self.Kty, self.Ktx)
         anchor, coef = self.window(u, v, row_scale)
         return GMatrix.from_window(ctx, anchor, coef, self.Kty, self.Ktx, self.Ky, self.Kx)
 

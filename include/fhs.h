@@ -77,6 +77,15 @@ int fhs_gmat_create_csr_host(fhs_ctx* ctx, int64_t M, int64_t ncols,
 int fhs_gmat_create_window_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx,
                                 int Ky, int Kx, const int32_t* anchor,
                                 const float* coef, fhs_gmat** out);
+/* Separable min-max G of a 2-D NUFFT (no w-term): G[m, (jy, jx)] =
+ * cy[m*Ky + jy] * cx[m*Kx + jx] on the wrapped window anchored at
+ * anchor[m] = ky0 + Kty*kx0 (interleaved complex, host pointers; any
+ * per-row scaling such as the noise whitening folded into cx).
+ * Ky, Kx <= 8, M < 2^26.  Same products as the window form, 3.5x less
+ * coefficient traffic; the adjoint CSC stores 4 B per nonzero. */
+int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, int Ky, int Kx,
+                                   const int32_t* anchor, const float* cy, const float* cx,
+                                   fhs_gmat** out);
 /* G straight from a MATLAB sparse matrix (CSC: mxGetJc / mxGetIr as
  * int64, interleaved complex values of the R2018a API, host pointers). */
 int fhs_gmat_create_csc_host(fhs_ctx* ctx, int64_t M, int64_t ncols,

@@ -135,6 +135,7 @@ class NufftOracle:
         # dense-window form: anchor = first tap (ky0 + Kty*kx0), taps jx*Ky + jy
         self.anchor = (iy[:, 0] + self.Kty * ix[:, 0]).astype(np.int64)
         self.coef = vals
+        self.cy_sep, self.cx_sep = cy, cx        # G[m, (jy, jx)] = cy[m, jy] cx[m, jx]
         self.Ky, self.Kx = Ky, Kx
 
     def A(self, x):

@@ -46,7 +46,9 @@ def test_update_dual_data_fidelity(ctx, precond, dims):
     rng = np.random.default_rng(M)
     ops, ys, pus, blocks = make(rng, Ny, Nx, L, M, nblk, precond)
     plan = fhs.NufftPlan(ctx, Ny, Nx, ops[0].Kty, ops[0].Ktx, ops[0].sy, ops[0].sx)
-    Gs = [fhs.GMatrix.from_window(ctx, op.anchor, op.coef, op.Kty, op.Ktx, op.Ky, op.Kx) for op in ops]
+    Gs = [fhs.GMatrix.from_separable(ctx, op.anchor, op.cy_sep, op.cx_sep, op.Kty, op.Ktx) if l % 2 else
+          fhs.GMatrix.from_window(ctx, op.anchor, op.coef, op.Kty, op.Ktx, op.Ky, op.Kx)
+          for l, op in enumerate(ops)]
     D = fhs.DataOperator(ctx, plan, Gs, [[b - a for a, b in bl] for bl in blocks])
     # oracle cells
     G = [[op.G[a:b] for a, b in bl] for op, bl in zip(ops, blocks)]

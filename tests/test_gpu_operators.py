@@ -34,7 +34,7 @@ def to_dev(a, dtype=None):
 
 
 @pytest.mark.parametrize('Ny,Nx,M', [(32, 32, 500), (64, 48, 3000), (256, 256, 40000)])
-@pytest.mark.parametrize('layout', ['csr', 'window'])
+@pytest.mark.parametrize('layout', ['csr', 'window', 'separable'])
 def test_direct_and_adjoint(ctx, Ny, Nx, M, layout):
     import fhs
     rng = np.random.default_rng(Ny * 7 + M)
@@ -42,9 +42,11 @@ def test_direct_and_adjoint(ctx, Ny, Nx, M, layout):
     op = NufftOracle(u, v, Ny, Nx)
     if layout == 'csr':
         G = fhs.GMatrix.from_scipy(ctx, op.G)
-    else:
+    elif layout == 'window':
         G = fhs.GMatrix.from_window(ctx, op.anchor, op.coef, op.Kty, op.Ktx, op.Ky, op.Kx)
-    Gr = op.G.astype(np.complex64).astype(np.complex128)   # the matrix the GPU holds
+    else:
+        G = fhs.GMatrix.from_separable(ctx, op.anchor, op.cy_sep, op.cx_sep, op.Kty, op.Ktx)
+    Gr = op.G.astype(np.complex64).astype(np.complex128)   # the matrix the GPU holds (+- FP32 products)
     Fx = crand(rng, op.No)
     y = fhs.apply_direct_operator(to_dev(Fx), G).cpu().numpy()
     assert rel(y, Gr @ Fx.astype(np.complex128)) < TOL
