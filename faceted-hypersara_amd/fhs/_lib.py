@@ -49,6 +49,9 @@ PROTOTYPES = {
     'fhs_apply_adjoint_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_nufft_create_host': [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p,
                               C.POINTER(c_void_p)],
+    'fhs_nufft_interp_coefficients': [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p, c_int, c_int,
+                                      c_int, c_int, c_double_p, c_double_p, c_void_p, c_void_p, c_void_p, c_i64,
+                                      c_void_p, c_void_p, c_void_p],
     'fhs_nufft_destroy': [c_void_p],
     'fhs_apply_scaled_fourier_transform': [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_apply_scaled_fourier_adjoint': [c_void_p, c_void_p, c_void_p, c_int, c_float_p, c_void_p],
@@ -89,6 +92,19 @@ PROTOTYPES = {
     'fhs_data_forward': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_data_adjoint': [c_void_p, c_void_p, c_void_p, c_void_p, c_float_p, c_void_p],
     'fhs_sara_update_weights': [c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p],
+    'fhs_nn_gemm': [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int,
+                    C.c_float, C.c_float, c_void_p, c_int],
+    'fhs_nn_im2col3': [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p],
+    'fhs_nn_col2im3': [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p],
+    'fhs_nn_relu_bwd': [c_void_p, c_void_p, c_void_p, c_i64],
+    'fhs_nn_maxpool2': [c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p],
+    'fhs_nn_maxpool2_bwd': [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p],
+    'fhs_nn_upsample2': [c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p],
+    'fhs_nn_upsample2_bwd': [c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p],
+    'fhs_nn_bias_grad': [c_void_p, c_void_p, c_int, c_i64, c_void_p],
+    'fhs_nn_mse': [c_void_p, c_void_p, c_void_p, c_i64, C.c_float, c_void_p, c_double_p],
+    'fhs_nn_adam': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, C.c_float, C.c_float, C.c_float,
+                    C.c_float, c_int],
     'fhs_update_primal': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                           c_double_p, c_double_p],
 }

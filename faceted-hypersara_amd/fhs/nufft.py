@@ -15,9 +15,13 @@ Kaiser-Bessel transform), the J coefficients of frequency w are
 origin to pixel N/2.  The coefficients are computed here once per
 visibility set; the iteration only reads them.
 """
-import numpy as np
+import ctypes as C
 
-from .core import GMatrix, NufftPlan
+import numpy as np
+import torch
+
+from ._lib import lib
+from .core import GMatrix, NufftPlan, _ptr
 
 KB_ALPHA_PER_J = 2.34
 SCALING_TERMS = 6
@@ -92,6 +96,28 @@ class Nufft2D:
         if row_scale is not None:
             cx = cx * np.asarray(row_scale)[:, None]
         return ay + self.Kty * ax, cy, cx
+
+    def separable_gpu(self, ctx, u, v, row_scale=None):
+        """``separable`` computed on the GPU (``fhs_nufft_interp_coefficients``,
+        FP64 arithmetic): anchor int32 (M,), cy (M, Ky), cx (M, Kx) complex64,
+        returned to the host for ``GMatrix.from_separable``."""
+        M = len(u)
+        dev = ctx.device
+        ud = torch.from_numpy(np.ascontiguousarray(u, dtype=np.float64)).to(dev)
+        vd = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
+        rs = None if row_scale is None else torch.from_numpy(
+            np.ascontiguousarray(np.broadcast_to(np.asarray(row_scale, dtype=np.float64), (M,)))).to(dev)
+        anchor = torch.empty(M, dtype=torch.int32, device=dev)
+        cy = torch.empty((M, self.Ky), dtype=torch.complex64, device=dev)
+        cx = torch.empty((M, self.Kx), dtype=torch.complex64, device=dev)
+        ay, ax = self.ay, self.ax
+        pa = [np.ascontiguousarray(t, dtype=np.float64) for t in (ay.alpha, ay.T, ax.alpha, ax.T)]
+        dp = [t.ctypes.data_as(C.POINTER(C.c_double)) for t in pa]
+        ctx.check(lib().fhs_nufft_interp_coefficients(
+            ctx.h, self.Ny, self.Kty, self.Ky, len(ay.alpha), dp[0], dp[1],
+            self.Nx, self.Ktx, self.Kx, len(ax.alpha), dp[2], dp[3],
+            _ptr(ud), _ptr(vd), _ptr(rs), M, _ptr(anchor), _ptr(cy), _ptr(cx)), 'nufft_interp_coefficients')
+        return anchor.cpu().numpy(), cy.cpu().numpy(), cx.cpu().numpy()
 
     def plan(self, ctx):
         return NufftPlan(ctx, self.Ny, self.Nx, self.Kty, self.Ktx, self.ay.s, self.ax.s)

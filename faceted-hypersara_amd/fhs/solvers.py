@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from ._lib import lib
-from .core import _ptr
+from .core import GMatrix, _ptr
 from .data_fidelity import DataOperator
 from .lowrank import LowRankPrior
 from .nufft import Nufft2D, uniform_weights
@@ -56,22 +56,23 @@ class Measurement:
         self.L, self.Ny, self.Nx = P.L, P.Ny, P.Nx
         self.nufft = Nufft2D(P.Ny, P.Nx)
         self.plan = self.nufft.plan(ctx)
+        Kty, Ktx = self.nufft.Kty, self.nufft.Ktx
+        # interpolation coefficients, computed once per channel on the GPU (anchor, cy, cx)
+        coef = [self.nufft.separable_gpu(ctx, P.u[l], P.v[l]) for l in range(P.L)]
         if not P.y:
-            Graw = [self.nufft.gmatrix(ctx, P.u[l], P.v[l]) for l in range(P.L)]
+            Graw = [GMatrix.from_separable(ctx, a, cy, cx, Kty, Ktx) for a, cy, cx in coef]
             Draw = DataOperator(ctx, self.plan, Graw, P.blocks)
             y0 = self._split(Draw, self._forward(Draw, _cube(P.X0)).cpu().numpy().astype(np.complex128))
             del Draw, Graw
             add_noise(P, y0)
         # Visibilities are reordered inside each block by the grid cell of
-        # their interpolation window: neighbouring rows then gather
-        # neighbouring grid points (forward) and the rows of a grid column
-        # sit close together (adjoint).  Per-block results are unchanged.
+        # their interpolation window (the anchor): neighbouring rows then
+        # gather neighbouring grid points (forward) and the rows of a grid
+        # column sit close together (adjoint).  Per-block results are unchanged.
         self.order = []
         for l in range(P.L):
             if sort_rows:
-                ay = np.mod(np.floor(P.v[l] / self.nufft.ay.gamma - 0.5 * self.nufft.Ky) + 1, self.nufft.Kty)
-                ax = np.mod(np.floor(P.u[l] / self.nufft.ax.gamma - 0.5 * self.nufft.Kx) + 1, self.nufft.Ktx)
-                key = (ay + self.nufft.Kty * ax).astype(np.int64)
+                key = coef[l][0]
                 b = np.concatenate([[0], np.cumsum(P.blocks[l])])
                 idx = np.concatenate([b[i] + np.argsort(key[b[i]:b[i + 1]], kind='stable')
                                       for i in range(len(P.blocks[l]))])
@@ -81,8 +82,12 @@ class Measurement:
         self.u = [P.u[l][self.order[l]] for l in range(P.L)]
         self.v = [P.v[l][self.order[l]] for l in range(P.L)]
         self.y_host = [np.asarray(P.y[l])[self.order[l]] for l in range(P.L)]
-        self.G = [self.nufft.gmatrix(ctx, self.u[l], self.v[l], row_scale=np.full(len(self.u[l]), 1.0 / P.sigma_noise[l]))
-                  for l in range(P.L)]
+        self.G = []
+        for l, (a, cy, cx) in enumerate(coef):
+            o = self.order[l]
+            self.G.append(GMatrix.from_separable(ctx, a[o], cy[o], cx[o] * np.float32(1.0 / P.sigma_noise[l]),
+                                                 Kty, Ktx))
+        del coef
         self.D = DataOperator(ctx, self.plan, self.G, P.blocks)
         self.y = torch.from_numpy(np.concatenate(self.y_host).astype(np.complex64)).cuda()
         if precond:
@@ -273,7 +278,12 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         lowrank = LowRankPrior(ctx, Ny, Nx, L)
     elif kind == 'fhs':
         lowrank = LowRankPrior(ctx, Ny, Nx, L, facets=facets)
-    xi = None if x_init is None else _cube(x_init)
+    if x_init is None:
+        xi = None
+    elif isinstance(x_init, torch.Tensor):          # (L, Nx, Ny) cube already on the GPU
+        xi = x_init.detach().to(device=ctx.device, dtype=torch.float32).reshape(L, Nx, Ny).contiguous()
+    else:
+        xi = _cube(x_init)
     eng = PdfbEngine(meas, sara, lowrank, xi)
     eng.set_graph(graph)
     eng.set_concurrent(concurrent)
@@ -310,7 +320,7 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     # initial weights (hs_initialize_*, fhs_initialize_dual_and_weights.m:88-112)
     Q = lowrank.Q if lowrank else 0
     w0 = np.ones((Q, L))
-    if x_init is not None and np.any(x_init):
+    if xi is not None and bool(torch.any(xi != 0)):
         _reweight(eng, alpha, sig, sig_bar)
         w0 = eng._w0
     thresh0 = None if lowrank is None else (beta0[:, None] * w0)
@@ -324,6 +334,11 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     info = {'rel_val': [], 'norm_res': [], 'norm_eps': [], 'snr': [], 'time': [], 'reweights': [],
             'tau': tau, 'sigma22': sigma22, 'sig': sig, 'sig_bar': sig_bar, 'nu2': nu2, 'son': son,
             'converged': False}
+    track = X0 is not None and (callback is not None or prm.get('track_snr', False))
+    if track:
+        X0d = X0 if isinstance(X0, torch.Tensor) else _cube(X0)
+        X0d = X0d.to(ctx.device).double()
+        nX0 = torch.linalg.vector_norm(X0d)
     t0 = time.perf_counter()
     t = 0
     for t in range(1, max_iter + 1):
@@ -334,8 +349,8 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         info['norm_res'].append(nres)
         info['norm_eps'].append(neps)
         info['time'].append(time.perf_counter() - t0)
-        if X0 is not None and (callback is not None or prm.get('track_snr', False)):
-            info['snr'].append(snr_db(X0, _uncube(eng.x)))
+        if track:
+            info['snr'].append(20 * np.log10((nX0 / torch.linalg.vector_norm(X0d - eng.x.double())).item()))
         if callback is not None and callback(t, eng, info) is False:
             break
         k = t - last_step

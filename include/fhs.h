@@ -109,6 +109,19 @@ int fhs_apply_adjoint_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* y,
 int fhs_nufft_create_host(fhs_ctx* ctx, int Ny, int Nx, int Kty, int Ktx,
                           const double* sy, const double* sx, fhs_nufft** out);
 int fhs_nufft_destroy(fhs_nufft* p);
+/* Min-max interpolation coefficients of M visibilities, computed on the GPU
+ * in FP64: the per-visibility loop of op_p_nufft that builds G
+ * (lib/operators/util_gen_measurement_operator_synth.m:85).  Per axis: image
+ * size N, grid K, J <= 8 taps, na <= 8 cosine-series scaling terms a[na]
+ * and the J x J matrix T (host arrays); y is paired with v, x with u.
+ * u, v, row_scale (or NULL): device double[M]; outputs (device):
+ * anchor int32[M] = (k0y+1 mod Kty) + Kty (k0x+1 mod Ktx), cy complex[M][Jy],
+ * cx complex[M][Jx] (times row_scale), so G(m, (jy, jx)) = cy[m][jy] cx[m][jx]
+ * (the layout of fhs_gmat_create_separable_host). */
+int fhs_nufft_interp_coefficients(fhs_ctx* ctx, int Ny, int Kty, int Jy, int nay, const double* ay,
+                                  const double* Ty, int Nx, int Ktx, int Jx, int nax, const double* ax,
+                                  const double* Tx, const double* u, const double* v,
+                                  const double* row_scale, int64_t M, int32_t* anchor, float* cy, float* cx);
 /* Fx(:, l) = A(x(:, :, l)), l = 1..L
  * src/apply_scaled_fourier_transform.m:21-24.  x: float[L][Nx][Ny];
  * Fx: complex[L][Kty*Ktx]. */
@@ -295,6 +308,35 @@ int fhs_pdfb_iterate(fhs_pdfb* p, int n_iter, double* scalars);
 /* last iteration's per-block residual norms ||G_b A x - y_b||: host double[nblocks] */
 int fhs_pdfb_norm_res(fhs_pdfb* p, double* norm_res);
 int fhs_pdfb_stream(fhs_pdfb* p, void** stream);
+
+/* ------------------------------------------- learned warm start (U-Net) -- */
+/* No reference counterpart: the initial cube the reference loads through
+ * xinit (imaging/imaging.m:881-891) is produced here by a U-Net trained
+ * with these kernels.  Activations FP32 [C][N][H][W]; GEMM operands rounded
+ * to BF16, FP32 accumulation on the matrix cores. */
+/* C = alpha op(A) op(B) + beta C (+ bias[m]) (ReLU); op(A) = A^T when
+ * trans_a (A stored K x M, leading dim lda), op(B) = B^T when trans_b. */
+int fhs_nn_gemm(fhs_ctx* ctx, int M, int N, int K, const float* A, int lda, int trans_a, const float* B,
+                int ldb, int trans_b, float* C, int ldc, float alpha, float beta, const float* bias, int relu);
+/* col[(c*9 + ky*3 + kx)][n,y,x] = in[c][n][y+ky-1][x+kx-1] (zero padding) */
+int fhs_nn_im2col3(fhs_ctx* ctx, const float* in, int C, int NB, int H, int W, float* col);
+/* adjoint of im2col3 (gather form, no atomics) */
+int fhs_nn_col2im3(fhs_ctx* ctx, const float* dcol, int C, int NB, int H, int W, float* din);
+/* g = 0 where y <= 0 */
+int fhs_nn_relu_bwd(fhs_ctx* ctx, const float* y, float* g, int64_t n);
+int fhs_nn_maxpool2(fhs_ctx* ctx, const float* in, int64_t planes, int H, int W, float* out);
+int fhs_nn_maxpool2_bwd(fhs_ctx* ctx, const float* in, const float* gout, int64_t planes, int H, int W,
+                        float* gin);
+int fhs_nn_upsample2(fhs_ctx* ctx, const float* in, int64_t planes, int H, int W, float* out);
+int fhs_nn_upsample2_bwd(fhs_ctx* ctx, const float* gout, int64_t planes, int H, int W, float* gin);
+/* db[c] = sum_q g[c][q] */
+int fhs_nn_bias_grad(fhs_ctx* ctx, const float* g, int C, int64_t Q, float* db);
+/* loss = sum (pred - target)^2 (host double), grad = grad_scale (pred - target) (NULL: none) */
+int fhs_nn_mse(fhs_ctx* ctx, const float* pred, const float* target, int64_t n, float grad_scale, float* grad,
+               double* loss);
+/* Adam step t (bias-corrected) */
+int fhs_nn_adam(fhs_ctx* ctx, float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+                float beta2, float eps, int step);
 
 #ifdef __cplusplus
 }

@@ -110,3 +110,39 @@ def test_update_primal(ctx, n, parts):
     assert rel(xd.cpu().numpy(), xr) < TOL
     assert rel(xhat.cpu().numpy(), xhr) < TOL
     assert abs(rel_x - rr) / rr < 1e-6 and abs(norm_x - nr) / nr < 1e-6
+
+
+@pytest.mark.parametrize('Ny,Nx', [(64, 64), (48, 80), (256, 256)])
+def test_interp_coefficients_gpu(ctx, Ny, Nx):
+    """GPU min-max coefficients (FP64 on the device) against the oracle's
+    restatement and the host builder: identical anchors, coefficients equal
+    to complex64 rounding, including on-grid and band-edge frequencies."""
+    from fhs.nufft import Nufft2D
+    rng = np.random.default_rng(Ny + Nx)
+    nf = Nufft2D(Ny, Nx)
+    u = rng.uniform(-np.pi, np.pi, 5000)
+    v = rng.uniform(-np.pi, np.pi, 5000)
+    # on-grid frequencies (exact Dirichlet zeros) and the band edges
+    u[:8] = 2 * np.pi * np.arange(-4, 4) / nf.Ktx
+    v[:8] = 2 * np.pi * np.arange(-4, 4) / nf.Kty
+    u[8:12] = [-np.pi, np.pi - 1e-12, 0.0, -1e-15]
+    v[8:12] = [np.pi - 1e-12, -np.pi, -1e-15, 0.0]
+    scale = rng.uniform(0.5, 2.0, 5000)
+    a, cy, cx = nf.separable_gpu(ctx, u, v, row_scale=scale)
+    ah, cyh, cxh = nf.separable(u, v, row_scale=scale)
+    assert np.array_equal(a, ah)
+    for g, h in ((cy, cyh), (cx, cxh)):
+        err = np.abs(g.astype(np.complex128) - h)
+        assert err.max() <= 2e-7 * np.abs(h).max()
+    # the product of the GPU factors against the oracle's G rows (unscaled)
+    orc = NufftOracle(u[:200], v[:200], Ny, Nx)
+    a0, cy0, cx0 = nf.separable_gpu(ctx, u[:200], v[:200])
+    Kty = nf.Kty
+    for m in range(0, 200, 37):
+        row = np.zeros(nf.No, dtype=np.complex128)
+        ky0, kx0 = a0[m] % Kty, a0[m] // Kty
+        for jx in range(nf.Kx):
+            for jy in range(nf.Ky):
+                row[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % nf.Ktx)] += cy0[m, jy] * cx0[m, jx]
+        ref = orc.G.getrow(m).toarray().ravel()
+        assert np.abs(row - ref).max() <= 1e-6 * np.abs(ref).max()
