@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, 'faceted-hypersara_amd')):
     if _p not in sys.path:
@@ -46,14 +48,23 @@ def algorithmic_bytes(meas, sara_s):
     """Per-launch algorithmic HBM bytes of the kernels for the layouts in use
     (complex = 8 B, real FP32 = 4 B; every operand read or written once).
     G is the separable min-max layout: Ky + Kx coefficients and an anchor
-    per row, 4-byte packed CSC entries for the adjoint."""
+    per row.  Its adjoint grids 32 x 32 tiles from per-tile row lists
+    (row index + local cell, 8 B per listed row): each row's coefficients,
+    v2 and pU counted once, each grid point written once."""
     L, No = meas.L, meas.nufft.No
+    nf = meas.nufft
     M = [g.M for g in meas.G]
-    nnz = [g.nnz for g in meas.G]
     N = meas.Ny * meas.Nx
-    row = 8 * (meas.nufft.Ky + meas.nufft.Kx) + 4
+    row = 8 * (nf.Ky + nf.Kx) + 4
     fwd = sum(row * m + 8 * No + 24 * m for m in M)                      # coefs, Fx, gx_old r/w, r2
-    adj = sum(4 * k + 12 * No + (row - 4 + 8) * m for k, m in zip(nnz, M))  # CSC, colptr+out, coefs+v2
+    adj = 0
+    for a in meas.anchor:
+        a = a.astype(np.int64)
+        ay, ax = a % nf.Kty, a // nf.Kty
+        ny = 1 + (ay // 32 != ((ay + nf.Ky - 1) % nf.Kty) // 32)
+        nx = 1 + (ax // 32 != ((ax + nf.Kx - 1) % nf.Ktx) // 32)
+        entries = int(np.sum(ny * nx))
+        adj += (row - 4 + 8 + 4) * len(a) + 8 * entries + 4 * (No // 1024 + 1) + 8 * No
     return {
         'data_adjoint': adj,
         'data_forward': fwd,

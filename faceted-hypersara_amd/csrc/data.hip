@@ -24,7 +24,8 @@ struct fhs_data {
   int nblocks = 0;
   int64_t Mtot = 0;
   int maxM = 0;
-  int adj_grid = 0;                    // max adjoint units + grid-tail units over channels
+  int adj_grid = 0;                    // max adjoint units + grid-tail units over CSC channels
+  int tile_grid = 0;                   // max tiles over tile-gridded channels
   int64_t No = 0;
   const fhs_nufft* A = nullptr;
   std::vector<int64_t> row_off;        // [L+1]
@@ -53,12 +54,15 @@ constexpr int NXCD = 8;   // MI355X: 8 XCDs, each with its own L2
 // NXCD * ceil(L / NXCD) * units workgroups and remapped so that XCD x only
 // runs channels x, x + 8, ...: one channel's v2 / coefficients / grid
 // region is then shared by one L2 instead of being pulled into all eight.
+// Units are interleaved over those channels (unit-major), so the first
+// units of every channel (the heaviest adjoint tiles) start first.
 __device__ __forceinline__ bool xcd_map(int L, int units, int& chan, int& unit) {
   const int b = blockIdx.x;
   const int xcd = b % NXCD, s = b / NXCD;
-  chan = xcd + NXCD * (s / units);
-  unit = s % units;
-  return chan < L;
+  const int cpx = (L + NXCD - 1) / NXCD;   // channels per XCD
+  chan = xcd + NXCD * (s % cpx);
+  unit = s / cpx;
+  return chan < L && unit < units;
 }
 
 inline unsigned xcd_grid(int L, int units) {
@@ -109,6 +113,9 @@ __global__ __launch_bounds__(256) void k_data_forward_dr(const GDev* __restrict_
 // B(y, eps) in the metric diag(pU), then the dual update and the residual.
 // z <- P_B(z - mu pU (z - w)), w = v2/pU + r2, mu = 1/max(pU), warm start
 // z = proj; stop after min_it once ||dz|| <= tol ||z|| or at max_it.
+// w is formed once and kept in the v2 buffer during the iterations (28 B
+// read + 8 B written per row and iteration instead of 44 B); the dual
+// update v2 + pU (r2 - z) is then pU (w - z).
 __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
     const int64_t* __restrict__ blk_start, const double* __restrict__ eps_arr,
     float2* __restrict__ v2, const float2* __restrict__ y, const float* __restrict__ pU,
@@ -120,6 +127,7 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
   const double eps = eps_arr[blockIdx.x];
   // max(pU) -> step
   float pmax = 0.f;
+#pragma unroll 4
   for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) pmax = fmaxf(pmax, pU ? pU[i] : 1.f);
   pmax = wave_max(pmax);
   if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = pmax;
@@ -129,13 +137,15 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
   const float pm = pmax > 0.f ? pmax : 1.f;
   // gradient step z - mu pU (z - w), mu = 1/max pU, as (1-a) z + a w with
   // a = pU / max pU (a == 1 exactly where pU is maximal, so a constant pU
-  // reproduces w bit for bit and the iteration stops at its fixed point)
-  // first gradient step norm
+  // reproduces w bit for bit and the iteration stops at its fixed point).
+  // Pass 1: w into v2, first gradient-step distance to y.
   double s1[1] = {0.0};
+#pragma unroll 4
   for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
     const float p = pU ? pU[i] : 1.f;
     const float2 vv = v2[i], rr = r2[i], zz = z[i], yy = y[i];
     const float2 w = make_float2(vv.x / p + rr.x, vv.y / p + rr.y);
+    v2[i] = w;
     const float aa = p / pm;
     const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
     const double dx = (double)g.x - yy.x, dy = (double)g.y - yy.y;
@@ -147,12 +157,12 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
     const double nrm = sqrt(s_cur);
     const float scale = nrm > 0.0 ? (float)fmin(eps / nrm, 1.0) : 1.f;
     double acc[3] = {0.0, 0.0, 0.0};  // ||dz||^2, ||z||^2, next ||g - y||^2
+#pragma unroll 4
     for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
       const float p = pU ? pU[i] : 1.f;
-      const float2 vv = v2[i], rr = r2[i], zz = z[i], yy = y[i];
-      const float2 w = make_float2(vv.x / p + rr.x, vv.y / p + rr.y);
+      const float2 w = v2[i], zz = z[i], yy = y[i];
       const float aa = p / pm;
-    const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
+      const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
       const float2 zn = make_float2(yy.x + (g.x - yy.x) * scale, yy.y + (g.y - yy.y) * scale);
       z[i] = zn;
       const double ddx = (double)zn.x - zz.x, ddy = (double)zn.y - zz.y;
@@ -167,12 +177,13 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
     if (it >= min_it && crit <= tol) break;
     s_cur = acc[2];
   }
-  // dual update and residual
+  // dual update v2 = pU (w - z) and residual ||G x - y||
   double res[1] = {0.0};
+#pragma unroll 4
   for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
     const float p = pU ? pU[i] : 1.f;
-    const float2 vv = v2[i], rr = r2[i], zz = z[i], yy = y[i], gg = gx[i];
-    v2[i] = make_float2(vv.x + p * (rr.x - zz.x), vv.y + p * (rr.y - zz.y));
+    const float2 w = v2[i], zz = z[i], yy = y[i], gg = gx[i];
+    v2[i] = make_float2(p * (w.x - zz.x), p * (w.y - zz.y));
     const double ex = (double)gg.x - yy.x, ey = (double)gg.y - yy.y;
     res[0] += ex * ex + ey * ey;
   }
@@ -210,9 +221,9 @@ __global__ __launch_bounds__(FB_THREADS) void k_l2_dual(const int64_t* __restric
   if (threadIdx.x == 0) norm_res[blockIdx.x] = acc[1];
 }
 
-// g2_l = G_l' (lam .* v2_l) (+ G_l (lam .* v2_l) in the DR form), one
-// workgroup per adjoint work unit of channel blockIdx.y; grid points past
-// a channel's columns are zero.
+// g2_l = G_l' (lam .* v2_l) (+ G_l (lam .* v2_l) in the DR form) for the
+// channels whose G keeps a CSC copy: one workgroup per adjoint work unit;
+// grid points past a channel's columns are zero.
 __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint(const GDev* __restrict__ gd,
                                                               const int64_t* __restrict__ row_off,
                                                               const float2* __restrict__ v2,
@@ -223,6 +234,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint(const GDev* __rest
   int l, u;
   if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
+  if (g.tiled) return;
   float2* out = g2 + (size_t)l * No;
   if (u >= g.nunits) {
     // zero the grid tail beyond the matrix columns (only when ncols < No)
@@ -238,18 +250,57 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint(const GDev* __rest
   const float2* v = v2 + row_off[l];
   const float* lm = lam ? lam + row_off[l] : nullptr;
   auto none = [](int) { return make_float2(0.f, 0.f); };
+  auto wl = [&](int r) { return cscale(v[r], lm[r]); };
+  auto wp = [&](int r) { return v[r]; };
   if (lm) {
-    auto w = [&](int r) { return cscale(v[r], lm[r]); };
     if (dr)
-      g_adjoint_unit(g, u, w, [&](int c) { return g_row_apply(g, c, w); }, out, prod);
+      g_adjoint_unit(g, u, wl, [&](int c) { return g_row_apply(g, c, wl); }, out, prod);
     else
-      g_adjoint_unit(g, u, w, none, out, prod);
+      g_adjoint_unit(g, u, wl, none, out, prod);
   } else {
-    auto w = [&](int r) { return v[r]; };
     if (dr)
-      g_adjoint_unit(g, u, w, [&](int c) { return g_row_apply(g, c, w); }, out, prod);
+      g_adjoint_unit(g, u, wp, [&](int c) { return g_row_apply(g, c, wp); }, out, prod);
     else
-      g_adjoint_unit(g, u, w, none, out, prod);
+      g_adjoint_unit(g, u, wp, none, out, prod);
+  }
+}
+
+// g2_l = G_l' (lam .* v2_l) for the channels with a tile-gridded separable G
+// (one workgroup per grid tile; never the DR form, whose H is square).
+template <bool LAM>
+__global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_tiled(const GDev* __restrict__ gd,
+                                                                    const int64_t* __restrict__ row_off,
+                                                                    const float2* __restrict__ v2,
+                                                                    const float* __restrict__ lam, int64_t No,
+                                                                    float2* __restrict__ g2, int L, int units) {
+  __shared__ TileLds t;
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
+  const GDev g = gd[l];
+  if (!g.tiled || u >= g.nunits) return;
+  const float2* __restrict__ v = v2 + row_off[l];
+  const float* __restrict__ lm = LAM ? lam + row_off[l] : nullptr;
+  if (LAM)
+    g_adjoint_tile(g, u, [=](int r) { return cscale(v[r], lm[r]); }, g2 + (size_t)l * No, t);
+  else
+    g_adjoint_tile(g, u, [=](int r) { return v[r]; }, g2 + (size_t)l * No, t);
+}
+
+void launch_data_adjoint(fhs_ctx* ctx, const fhs_data* D, const float2* v2, const float* lam, int dr) {
+  const int L = D->L;
+  if (D->tile_grid > 0) {
+    if (lam)
+      k_data_adjoint_tiled<true><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, 0, ctx->stream>>>(
+          D->gdev, D->d_row_off, v2, lam, D->No, D->g2, L, D->tile_grid);
+    else
+      k_data_adjoint_tiled<false><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, 0, ctx->stream>>>(
+          D->gdev, D->d_row_off, v2, nullptr, D->No, D->g2, L, D->tile_grid);
+    check_launch();
+  }
+  if (D->adj_grid > 0) {
+    k_data_adjoint<<<xcd_grid(L, D->adj_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr,
+                                                                              D->No, D->g2, L, D->adj_grid);
+    check_launch();
   }
 }
 
@@ -323,9 +374,7 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
   }
   {
     Scope sc(ctx, "data_adjoint");
-    k_data_adjoint<<<xcd_grid(L, D->adj_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr,
-                                                                              D->No, D->g2, L, D->adj_grid);
-    check_launch();
+    launch_data_adjoint(ctx, D, v2, lam, dr);
   }
   nufft_adjoint(ctx, D->A, D->g2, L, D->d_sigma22, Ftx);
   if (d_out2) {
@@ -371,9 +420,7 @@ void data_adjoint(fhs_ctx* ctx, fhs_data* D, const float2* v, const float* lam, 
                   float* out) {
   {
     Scope sc(ctx, "data_adjoint");
-    k_data_adjoint<<<xcd_grid(D->L, D->adj_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v, lam, 0,
-                                                                                 D->No, D->g2, D->L, D->adj_grid);
-    check_launch();
+    launch_data_adjoint(ctx, D, v, lam, 0);
   }
   nufft_adjoint(ctx, D->A, D->g2, D->L, scale_dev, out);
 }
@@ -387,6 +434,7 @@ void data_get_norm_res(fhs_ctx* ctx, fhs_data* D, double* h_out) {
 }
 
 int data_nblocks(const fhs_data* D) { return D->nblocks; }
+bool data_square(const fhs_data* D) { return D->square; }
 int data_channels(const fhs_data* D) { return D->L; }
 
 }  // namespace fhs
@@ -418,9 +466,13 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
       gd[l] = G[l]->d;
       D->row_off[l + 1] = D->row_off[l] + G[l]->M;
       D->maxM = std::max<int>(D->maxM, (int)G[l]->M);
-      const int64_t tail = (D->No - G[l]->ncols + ADJ_COLS - 1) / ADJ_COLS;
-      D->adj_grid = std::max<int>(D->adj_grid, G[l]->d.nunits + (int)tail);
-      square = square && (G[l]->ncols == G[l]->M);
+      if (G[l]->d.tiled) {
+        D->tile_grid = std::max<int>(D->tile_grid, G[l]->d.nunits);
+      } else {
+        const int64_t tail = (D->No - G[l]->ncols + ADJ_COLS - 1) / ADJ_COLS;
+        D->adj_grid = std::max<int>(D->adj_grid, G[l]->d.nunits + (int)tail);
+      }
+      square = square && (G[l]->ncols == G[l]->M) && !G[l]->d.tiled;   // H of the DR form is not tiled
     }
     D->square = square;
     D->Mtot = D->row_off[L];

@@ -55,6 +55,60 @@ std::vector<int> adjoint_units(const std::vector<int>& colptr) {
   return u;
 }
 
+struct TileLists {
+  int tiles_y = 0, tiles_x = 0;
+  std::vector<int> ptr, rows, cell, order;
+};
+
+// every row under each tile its Ky x Kx window touches, sorted by local cell
+TileLists tile_lists(int64_t M, const int32_t* anchor, int Kty, int Ktx, int Ky, int Kx) {
+  TileLists T;
+  T.tiles_y = Kty / TILE_Y;
+  T.tiles_x = Ktx / TILE_X;
+  const int nt = T.tiles_y * T.tiles_x, ncy = TILE_Y + Ky - 1;
+  auto visit = [&](int a, auto&& f) {
+    const int ay = a % Kty, ax = a / Kty;
+    const int ys[2] = {ay / TILE_Y, ((ay + Ky - 1) % Kty) / TILE_Y};
+    const int xs[2] = {ax / TILE_X, ((ax + Kx - 1) % Ktx) / TILE_X};
+    for (int i = 0; i < 2; ++i) {
+      if (i == 1 && xs[1] == xs[0]) break;
+      for (int j = 0; j < 2; ++j) {
+        if (j == 1 && ys[1] == ys[0]) break;
+        const int ty = ys[j], tx = xs[i];
+        const int cyl = ((ay - ty * TILE_Y + Ky - 1) % Kty + Kty) % Kty;
+        const int cxl = ((ax - tx * TILE_X + Kx - 1) % Ktx + Ktx) % Ktx;
+        f(tx * T.tiles_y + ty, cxl * ncy + cyl);
+      }
+    }
+  };
+  std::vector<int64_t> cnt(nt + 1, 0);
+  for (int64_t m = 0; m < M; ++m) visit(anchor[m], [&](int t, int) { cnt[t + 1]++; });
+  for (int t = 0; t < nt; ++t) cnt[t + 1] += cnt[t];
+  FHS_CHECK(cnt[nt] < (1ll << 31), "gmat_create_separable: tile lists too long");
+  std::vector<std::pair<int, int>> ent(cnt[nt]);   // (cell, row)
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t m = 0; m < M; ++m)
+    visit(anchor[m], [&](int t, int c) { ent[pos[t]++] = {c, (int)m}; });
+  T.ptr.resize(nt + 1);
+  T.rows.resize(cnt[nt]);
+  T.cell.resize(cnt[nt]);
+  for (int t = 0; t < nt; ++t) {
+    std::sort(ent.begin() + cnt[t], ent.begin() + cnt[t + 1]);   // by cell, then row
+    T.ptr[t] = (int)cnt[t];
+  }
+  T.ptr[nt] = (int)cnt[nt];
+  for (size_t e = 0; e < ent.size(); ++e) {
+    T.cell[e] = ent[e].first;
+    T.rows[e] = ent[e].second;
+  }
+  // heaviest tiles are dispatched first so that they do not form the tail
+  T.order.resize(nt);
+  std::iota(T.order.begin(), T.order.end(), 0);
+  std::stable_sort(T.order.begin(), T.order.end(),
+                   [&](int a, int b) { return cnt[a + 1] - cnt[a] > cnt[b + 1] - cnt[b]; });
+  return T;
+}
+
 __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
                                const float* __restrict__ lam, float2* __restrict__ y) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,12 +121,19 @@ __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
 __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint(GDev g, const float2* __restrict__ y,
                                                                const float* __restrict__ lam,
                                                                float2* __restrict__ out) {
-  __shared__ float2 prod[ADJ_LDS];
+  __shared__ __align__(16) unsigned char lds[ADJ_LDS_BYTES];
   auto none = [](int) { return make_float2(0.f, 0.f); };
-  if (lam)
-    g_adjoint_unit(g, blockIdx.x, [&](int r) { return cscale(y[r], lam[r]); }, none, out, prod);
-  else
-    g_adjoint_unit(g, blockIdx.x, [&](int r) { return y[r]; }, none, out, prod);
+  auto wl = [&](int r) { return cscale(y[r], lam[r]); };
+  auto wp = [&](int r) { return y[r]; };
+  if (g.tiled) {
+    TileLds& t = *reinterpret_cast<TileLds*>(lds);
+    if (lam) g_adjoint_tile(g, blockIdx.x, wl, out, t);
+    else g_adjoint_tile(g, blockIdx.x, wp, out, t);
+  } else {
+    float2* prod = reinterpret_cast<float2*>(lds);
+    if (lam) g_adjoint_unit(g, blockIdx.x, wl, none, out, prod);
+    else g_adjoint_unit(g, blockIdx.x, wp, none, out, prod);
+  }
 }
 
 }  // namespace
@@ -232,33 +293,44 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
     const int K = Ky * Kx;
     const int64_t nnz = M * K;
     FHS_CHECK(nnz < (1ll << 31), "gmat_create_separable: nnz out of range");
-    std::vector<float2> sep((size_t)(Ky + Kx) * M);
+    std::vector<float2> sep((size_t)SEP_ROW * M, make_float2(0.f, 0.f));
     std::vector<int> anc(anchor, anchor + M);
-    std::vector<int> colptr(ncols + 1, 0);
     for (int64_t m = 0; m < M; ++m) {
       const int a = anchor[m];
       FHS_CHECK(a >= 0 && a < ncols, "gmat_create_separable: anchor out of range");
       for (int jy = 0; jy < Ky; ++jy)
-        sep[(size_t)jy * M + m] = make_float2(cy[2 * (m * Ky + jy)], cy[2 * (m * Ky + jy) + 1]);
+        sep[(size_t)m * SEP_ROW + jy] = make_float2(cy[2 * (m * Ky + jy)], cy[2 * (m * Ky + jy) + 1]);
       for (int jx = 0; jx < Kx; ++jx)
-        sep[(size_t)(Ky + jx) * M + m] = make_float2(cx[2 * (m * Kx + jx)], cx[2 * (m * Kx + jx) + 1]);
-      const int ky0 = a % Kty, kx0 = a / Kty;
-      for (int jx = 0; jx < Kx; ++jx)
-        for (int jy = 0; jy < Ky; ++jy) colptr[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx) + 1]++;
+        sep[(size_t)m * SEP_ROW + SEP_KMAX + jx] = make_float2(cx[2 * (m * Kx + jx)], cx[2 * (m * Kx + jx) + 1]);
     }
-    for (int64_t k = 0; k < ncols; ++k) colptr[k + 1] += colptr[k];
-    std::vector<int> pos(colptr.begin(), colptr.end() - 1);
-    std::vector<unsigned> pk(nnz);
-    for (int64_t m = 0; m < M; ++m) {   // rows ascending inside every column
-      const int a = anchor[m];
-      const int ky0 = a % Kty, kx0 = a / Kty;
-      for (int jx = 0; jx < Kx; ++jx)
-        for (int jy = 0; jy < Ky; ++jy) {
-          const int col = (ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx);
-          pk[pos[col]++] = ((unsigned)m << 6) | (unsigned)(jx * Ky + jy);
-        }
+    const bool tiled = tile_grid_ok(Kty, Ktx, Ky, Kx);
+    std::vector<int> colptr, units;
+    std::vector<unsigned> pk;
+    TileLists tl;
+    if (tiled) {
+      tl = tile_lists(M, anchor, Kty, Ktx, Ky, Kx);
+    } else {
+      colptr.assign(ncols + 1, 0);
+      for (int64_t m = 0; m < M; ++m) {
+        const int a = anchor[m];
+        const int ky0 = a % Kty, kx0 = a / Kty;
+        for (int jx = 0; jx < Kx; ++jx)
+          for (int jy = 0; jy < Ky; ++jy) colptr[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx) + 1]++;
+      }
+      for (int64_t k = 0; k < ncols; ++k) colptr[k + 1] += colptr[k];
+      std::vector<int> pos(colptr.begin(), colptr.end() - 1);
+      pk.resize(nnz);
+      for (int64_t m = 0; m < M; ++m) {   // rows ascending inside every column
+        const int a = anchor[m];
+        const int ky0 = a % Kty, kx0 = a / Kty;
+        for (int jx = 0; jx < Kx; ++jx)
+          for (int jy = 0; jy < Ky; ++jy) {
+            const int col = (ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx);
+            pk[pos[col]++] = ((unsigned)m << 6) | (unsigned)(jx * Ky + jy);
+          }
+      }
+      units = adjoint_units(colptr);
     }
-    const std::vector<int> units = adjoint_units(colptr);
     FHS_HIP(hipSetDevice(ctx->device));
     fhs_gmat* g = new fhs_gmat();
     g->device = ctx->device;
@@ -267,9 +339,6 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
     g->nnz = nnz;
     g->mem[0] = upload(anc);
     g->mem[1] = upload(sep);
-    g->mem[2] = upload(colptr);
-    g->mem[3] = upload(pk);
-    g->mem[6] = upload(units);
     GDev& d = g->d;
     d.layout = G_SEP;
     d.M = (int)M;
@@ -281,10 +350,28 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
     d.ncols = (int)ncols;
     d.anchor = (const int*)g->mem[0];
     d.vals = (const float2*)g->mem[1];
-    d.colptr = (const int*)g->mem[2];
-    d.csc_pk = (const unsigned*)g->mem[3];
-    d.ucol = (const int*)g->mem[6];
-    d.nunits = (int)units.size() - 1;
+    if (tiled) {
+      g->mem[7] = upload(tl.ptr);
+      g->mem[8] = upload(tl.rows);
+      g->mem[9] = upload(tl.cell);
+      g->mem[4] = upload(tl.order);
+      d.tiled = 1;
+      d.tiles_y = tl.tiles_y;
+      d.tiles_x = tl.tiles_x;
+      d.tile_ptr = (const int*)g->mem[7];
+      d.tile_rows = (const int*)g->mem[8];
+      d.tile_cell = (const int*)g->mem[9];
+      d.tile_order = (const int*)g->mem[4];
+      d.nunits = tl.tiles_y * tl.tiles_x;
+    } else {
+      g->mem[2] = upload(colptr);
+      g->mem[3] = upload(pk);
+      g->mem[6] = upload(units);
+      d.colptr = (const int*)g->mem[2];
+      d.csc_pk = (const unsigned*)g->mem[3];
+      d.ucol = (const int*)g->mem[6];
+      d.nunits = (int)units.size() - 1;
+    }
     *out = g;
   });
 }

@@ -10,12 +10,15 @@
 //            window, no column indices (8 B per nonzero instead of 12).
 //   SEP    : the min-max interpolator of a 2-D NUFFT is the Kronecker
 //            product of two 1-D interpolators, G[m, (jy, jx)] =
-//            cy[m][jy] cx[m][jx]: a row stores Ky + Kx complex values
-//            (112 B for 7 x 7 instead of 392 B) plus its anchor.  The CSC of
-//            the adjoint then stores only (row << 6 | tap) per nonzero (4 B).
-// Adjoint (gridding, Fx = G' y) is computed per grid point from a CSC copy
-// (colptr / rows / conj-ready values): each output is owned by one thread,
-// no atomics, deterministic summation order.
+//            cy[m][jy] cx[m][jx]: a row stores Ky + Kx complex values plus
+//            its anchor, row-major in one 128-byte line (cy in slots 0-7, cx
+//            in slots 8-15, zero padded) instead of 392 B for 7 x 7 taps, so
+//            any gather order costs one line per row.  The CSC of the adjoint
+//            (small grids) stores only (row << 6 | tap) per nonzero (4 B).
+// Adjoint (gridding, Fx = G' y) is computed per grid point, each output
+// owned by one thread (no atomics, deterministic summation order): from a
+// CSC copy (colptr / rows / values) in general, by tile gridding for the
+// separable layout on grids of whole 32 x 32 tiles.
 #pragma once
 
 #include "common.hpp"
@@ -24,6 +27,7 @@ namespace fhs {
 
 enum GLayout { G_ELL = 0, G_WINDOW = 1, G_SEP = 2 };
 constexpr int SEP_KMAX = 8;   // max taps per axis of the separable layout
+constexpr int SEP_ROW = 2 * SEP_KMAX;   // complex slots per separable row (128 B)
 
 struct GDev {
   int layout;
@@ -32,14 +36,33 @@ struct GDev {
   int ncols;
   const int* cols;      // ELL: [K][M]
   const int* anchor;    // WINDOW: [M]
-  const float2* vals;   // [K][M]
+  const float2* vals;   // ELL / WINDOW: [K][M]; SEP: [M][SEP_ROW]
   const int* colptr;    // [ncols + 1]
   const int* csc_rows;  // [nnz]
   const float2* csc_vals;
   const int* ucol;      // adjoint work units: unit u = columns [ucol[u], ucol[u+1])
   int nunits;
   const unsigned* csc_pk;  // SEP: (row << 6) | (jx*Ky + jy) per nonzero
+  // SEP on a tiled grid: adjoint by tile gridding (no CSC; nunits = tiles)
+  int tiled, tiles_y, tiles_x;
+  const int* tile_ptr;     // [tiles + 1]
+  const int* tile_rows;    // rows listed under every tile their window touches
+  const int* tile_cell;    // local anchor cell of each listed row
+  const int* tile_order;   // tiles by decreasing row count (longest first)
 };
+
+// the 16 complex slots of separable row m (8 x 16-byte loads)
+__device__ __forceinline__ void sep_load(const GDev& g, int m, float2* cy, float2* cx) {
+  const float4* p = reinterpret_cast<const float4*>(g.vals + (size_t)m * SEP_ROW);
+#pragma unroll
+  for (int i = 0; i < SEP_KMAX / 2; ++i) {
+    const float4 a = p[i], b = p[SEP_KMAX / 2 + i];
+    cy[2 * i] = make_float2(a.x, a.y);
+    cy[2 * i + 1] = make_float2(a.z, a.w);
+    cx[2 * i] = make_float2(b.x, b.y);
+    cx[2 * i + 1] = make_float2(b.z, b.w);
+  }
+}
 
 // value and row of CSC entry e
 __device__ __forceinline__ void g_csc_entry(const GDev& g, int e, float2& v, int& r) {
@@ -47,7 +70,7 @@ __device__ __forceinline__ void g_csc_entry(const GDev& g, int e, float2& v, int
     const unsigned pk = g.csc_pk[e];
     r = (int)(pk >> 6);
     const int tap = (int)(pk & 63u), jx = tap / g.Ky, jy = tap - jx * g.Ky;
-    v = cmul(g.vals[(size_t)jy * g.M + r], g.vals[(size_t)(g.Ky + jx) * g.M + r]);
+    v = cmul(g.vals[(size_t)r * SEP_ROW + jy], g.vals[(size_t)r * SEP_ROW + SEP_KMAX + jx]);
   } else {
     v = g.csc_vals[e];
     r = g.csc_rows[e];
@@ -77,10 +100,8 @@ __device__ __forceinline__ float2 g_row_apply(const GDev& g, int m, F&& f) {
   } else if (g.layout == G_SEP) {
     const int a = g.anchor[m];
     const int ky0 = a % g.Kty, kx0 = a / g.Kty;
-    float2 cy[SEP_KMAX];
-#pragma unroll
-    for (int jy = 0; jy < SEP_KMAX; ++jy)
-      cy[jy] = jy < g.Ky ? g.vals[(size_t)jy * g.M + m] : make_float2(0.f, 0.f);
+    float2 cy[SEP_KMAX], cx[SEP_KMAX];
+    sep_load(g, m, cy, cx);
     for (int jx = 0; jx < g.Kx; ++jx) {
       int kx = kx0 + jx;
       if (kx >= g.Ktx) kx -= g.Ktx;
@@ -96,7 +117,7 @@ __device__ __forceinline__ float2 g_row_apply(const GDev& g, int m, F&& f) {
           s.y += cy[jy].x * x.y + cy[jy].y * x.x;
         }
       }
-      const float2 c = g.vals[(size_t)(g.Ky + jx) * g.M + m];
+      const float2 c = cx[jx];
       acc.x += c.x * s.x - c.y * s.y;
       acc.y += c.x * s.y + c.y * s.x;
     }
@@ -171,11 +192,107 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
   }
 }
 
+// Tile-gridding adjoint of the separable layout.
+//
+// The grid is cut into TILE_Y x TILE_X tiles.  At creation every row is
+// listed under each tile its Ky x Kx window touches (1, 2 or 4 tiles), and
+// each tile's list is sorted by the row's anchor cell relative to the tile
+// ("local cell": (TILE_Y + Ky - 1) x (TILE_X + Kx - 1) cells, x-major).  One
+// workgroup owns one tile: it stages 256 listed rows at a time in LDS as
+//   Q[k][jx] = conj(cx[jx]) w(row),   C[k][jy] = conj(cy[jy]),
+// finds where each local cell starts in the staged chunk (binary search), and
+// every thread then sums, for the grid points it owns, the rows of the Ky
+// consecutive cells of each of the Kx cell columns that reach the point.
+// Each output point is written once by one thread, in a fixed order: no
+// atomics, deterministic, no per-nonzero index stream (the CSC is not built).
+constexpr int TILE_Y = 32, TILE_X = 32;
+constexpr int TILE_CH = ADJ_THREADS;                 // rows staged per chunk
+constexpr int TILE_PTS = TILE_Y * TILE_X / ADJ_THREADS;
+constexpr int TILE_CELLS_MAX = (TILE_Y + SEP_KMAX - 1) * (TILE_X + SEP_KMAX - 1);
+
+struct TileLds {
+  float2 Q[TILE_CH][SEP_KMAX];
+  float2 C[TILE_CH][SEP_KMAX];
+  int cell[TILE_CH];
+  int start[TILE_CELLS_MAX + 1];
+};
+
+// LDS of an adjoint workgroup: the CSC unit stage or the tile stage
+constexpr size_t ADJ_LDS_BYTES =
+    sizeof(TileLds) > ADJ_LDS * sizeof(float2) ? sizeof(TileLds) : ADJ_LDS * sizeof(float2);
+
+inline bool tile_grid_ok(int Kty, int Ktx, int Ky, int Kx) {
+  return Kty % TILE_Y == 0 && Ktx % TILE_X == 0 && Kty >= TILE_Y + Ky - 1 && Ktx >= TILE_X + Kx - 1;
+}
+
+template <class W>
+__device__ __forceinline__ void g_adjoint_tile(const GDev& g, int t, W&& w, float2* __restrict__ out,
+                                               TileLds& s) {
+  t = g.tile_order[t];
+  const int ty = t % g.tiles_y, tx = t / g.tiles_y;
+  const int ncy = TILE_Y + g.Ky - 1, ncx = TILE_X + g.Kx - 1, ncells = ncy * ncx;
+  const int p0 = g.tile_ptr[t], p1 = g.tile_ptr[t + 1];
+  float2 acc[TILE_PTS];
+#pragma unroll
+  for (int i = 0; i < TILE_PTS; ++i) acc[i] = make_float2(0.f, 0.f);
+  for (int c0 = p0; c0 < p1; c0 += TILE_CH) {
+    const int n = min(TILE_CH, p1 - c0);
+    const int i = threadIdx.x;
+    if (i < n) {
+      const int r = g.tile_rows[c0 + i];
+      const float2 v = w(r);
+      float2 cy[SEP_KMAX], cx[SEP_KMAX];
+      sep_load(g, r, cy, cx);
+#pragma unroll
+      for (int j = 0; j < SEP_KMAX; ++j) {
+        const float2 c = cx[j];
+        s.Q[i][j] = make_float2(c.x * v.x + c.y * v.y, c.x * v.y - c.y * v.x);   // conj(c) v
+        s.C[i][j] = make_float2(cy[j].x, -cy[j].y);
+      }
+      s.cell[i] = g.tile_cell[c0 + i];
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c <= ncells; c += ADJ_THREADS) {   // first staged row with cell >= c
+      int lo = 0, hi = n;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s.cell[mid] < c) lo = mid + 1; else hi = mid;
+      }
+      s.start[c] = lo;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TILE_PTS; ++q) {
+      const int p = threadIdx.x + q * ADJ_THREADS;
+      const int ly = p % TILE_Y, lx = p / TILE_Y;
+      float2 a = acc[q];
+      for (int jx = 0; jx < g.Kx; ++jx) {
+        const int base = (lx + g.Kx - 1 - jx) * ncy;
+        const int k0 = s.start[base + ly], k1 = s.start[base + ly + g.Ky];
+        for (int k = k0; k < k1; ++k) {
+          const int jy = ly + g.Ky - 1 - (s.cell[k] - base);
+          const float2 cy = s.C[k][jy], qx = s.Q[k][jx];
+          a.x += cy.x * qx.x - cy.y * qx.y;
+          a.y += cy.x * qx.y + cy.y * qx.x;
+        }
+      }
+      acc[q] = a;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < TILE_PTS; ++q) {
+    const int p = threadIdx.x + q * ADJ_THREADS;
+    const int ly = p % TILE_Y, lx = p / TILE_Y;
+    out[(size_t)(tx * TILE_X + lx) * g.Kty + ty * TILE_Y + ly] = acc[q];
+  }
+}
+
 }  // namespace fhs
 
 struct fhs_gmat {
   int device = 0;
   fhs::GDev d{};
   int64_t M = 0, ncols = 0, nnz = 0;
-  void* mem[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* mem[10] = {};
 };

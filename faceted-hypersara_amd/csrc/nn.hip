@@ -9,6 +9,7 @@
 //   grad weight  dW[co][k]   = sum_q dout[co][q] col[k][q]
 // with k = ci*9 + ky*3 + kx and q = (n, y, x).  Operands are rounded to BF16
 // when staged in LDS; v_mfma_f32_16x16x16_bf16 accumulates in FP32.
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -31,13 +32,18 @@ constexpr int GTHREADS = 256;
 
 // C (M x N, row-major ldc) = alpha op(A) op(B) + beta C (+ bias[m]) (ReLU)
 // op(A)[m][k] = ta ? A[k*lda + m] : A[m*lda + k]; op(B)[k][n] = tb ? B[n*ldb + k] : B[k*ldb + n]
+// Split-K: workgroup z sums k in [z*kchunk, (z+1)*kchunk) and, when part is
+// given, stores alpha * (its partial product) to part[z][M][N]; k_gemm_reduce
+// then adds the partials in z order (deterministic), beta C, bias and ReLU.
 __global__ __launch_bounds__(GTHREADS) void k_gemm_bf16(int M, int N, int K, const float* __restrict__ A,
                                                         int lda, int ta, const float* __restrict__ B, int ldb,
                                                         int tb, float* __restrict__ Cm, int ldc, float alpha,
-                                                        float beta, const float* __restrict__ bias, int relu) {
+                                                        float beta, const float* __restrict__ bias, int relu,
+                                                        int kchunk, float* __restrict__ part) {
   __shared__ unsigned short As[GT][GK + 4];
   __shared__ unsigned short Bs[GK][GT + 4];
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;   // wave sub-tile origin
   v4f acc[2][2];
@@ -45,14 +51,14 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_bf16(int M, int N, int K, con
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  for (int k0 = kb; k0 < ke; k0 += GK) {
     for (int t = threadIdx.x; t < GT * GK; t += GTHREADS) {
       // A tile: GT rows x GK cols
       int r, c;
       if (ta) { r = t % GT; c = t / GT; } else { c = t % GK; r = t / GK; }
       const int m = m0 + r, k = k0 + c;
       float v = 0.f;
-      if (m < M && k < K) v = ta ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+      if (m < M && k < ke) v = ta ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
       As[r][c] = to_bf16(v);
     }
     for (int t = threadIdx.x; t < GT * GK; t += GTHREADS) {
@@ -60,7 +66,7 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_bf16(int M, int N, int K, con
       if (tb) { r = t % GK; c = t / GK; } else { c = t % GT; r = t / GT; }
       const int k = k0 + r, n = n0 + c;
       float v = 0.f;
-      if (n < N && k < K) v = tb ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
+      if (n < N && k < ke) v = tb ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
       Bs[r][c] = to_bf16(v);
     }
     __syncthreads();
@@ -96,12 +102,30 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_bf16(int M, int N, int K, con
         const int n = n0 + wn + 16 * j + (lane & 15);
         if (m < M && n < N) {
           float v = alpha * acc[i][j][r];
+          if (part) {
+            part[((size_t)blockIdx.z * M + m) * N + n] = v;
+            continue;
+          }
           if (beta != 0.f) v += beta * Cm[(size_t)m * ldc + n];
           if (bias) v += bias[m];
           if (relu) v = fmaxf(v, 0.f);
           Cm[(size_t)m * ldc + n] = v;
         }
       }
+}
+
+__global__ void k_gemm_reduce(int M, int N, int splits, const float* __restrict__ part, float* __restrict__ Cm,
+                              int ldc, float beta, const float* __restrict__ bias, int relu) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / N), n = (int)(i % N);
+    float v = 0.f;
+    for (int z = 0; z < splits; ++z) v += part[(size_t)z * total + i];
+    if (beta != 0.f) v += beta * Cm[(size_t)m * ldc + n];
+    if (bias) v += bias[m];
+    if (relu) v = fmaxf(v, 0.f);
+    Cm[(size_t)m * ldc + n] = v;
+  }
 }
 
 // col[ci*9 + ky*3 + kx][n, y, x] = in[ci][n][y + ky - 1][x + kx - 1]
@@ -255,9 +279,26 @@ int fhs_nn_gemm(fhs_ctx* ctx, int M, int N, int K, const float* A, int lda, int 
   return guard(ctx, [&] {
     FHS_CHECK(M > 0 && N > 0 && K > 0 && A && B && C, "nn_gemm: bad argument");
     Scope sc(ctx, "nn_gemm");
-    k_gemm_bf16<<<dim3(cdiv(N, GT), cdiv(M, GT)), GTHREADS, 0, ctx->stream>>>(
-        M, N, K, A, lda, trans_a, B, ldb, trans_b, C, ldc, alpha, beta, bias, relu);
+    // long K over few output tiles (the weight gradient): split K across
+    // workgroups so that the chip fills, partials reduced in a fixed order
+    const int64_t tiles = (int64_t)cdiv(N, GT) * cdiv(M, GT);
+    int splits = 1;
+    if (tiles < 1024 && K >= 4 * 1024) {
+      splits = (int)std::min<int64_t>(cdiv(K, 1024), std::max<int64_t>(1, 2048 / tiles));
+      splits = std::min(splits, 1024);
+    }
+    const int kchunk = (int)((cdiv(K, splits) + GK - 1) / GK * GK);
+    splits = (int)cdiv(K, kchunk);
+    float* part = nullptr;
+    if (splits > 1)
+      part = (float*)ctx->ws.get("nn_gemm_split", (size_t)splits * M * N * sizeof(float));
+    k_gemm_bf16<<<dim3(cdiv(N, GT), cdiv(M, GT), splits), GTHREADS, 0, ctx->stream>>>(
+        M, N, K, A, lda, trans_a, B, ldb, trans_b, C, ldc, alpha, beta, bias, relu, kchunk, part);
     check_launch();
+    if (splits > 1) {
+      k_gemm_reduce<<<egrid((int64_t)M * N), 256, 0, ctx->stream>>>(M, N, splits, part, C, ldc, beta, bias, relu);
+      check_launch();
+    }
   });
 }
 

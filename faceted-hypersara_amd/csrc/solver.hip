@@ -34,6 +34,7 @@ void data_set_epsilon(fhs_ctx* ctx, fhs_data* D, const double* h_eps);
 void data_get_norm_res(fhs_ctx* ctx, fhs_data* D, double* h_out);
 int data_nblocks(const fhs_data* D);
 int data_channels(const fhs_data* D);
+bool data_square(const fhs_data* D);
 int lowrank_facets(const fhs_lowrank* R);
 int lowrank_channels(const fhs_lowrank* R);
 }  // namespace fhs
@@ -182,6 +183,8 @@ int fhs_pdfb_set_params(fhs_pdfb* P, const double* thresh0, double beta1, float 
         for (int i = 0; i < P->L; ++i) th[(size_t)q * 16 + i] = thresh0[(size_t)q * P->L + i];
       lowrank_set_thresh(c, P->R, th.data());
     }
+    FHS_CHECK(!flag_dimensionality_reduction || data_square(P->D),
+              "pdfb_set_params: DR needs square (holographic) G");
     data_set_sigma22(c, P->D, sigma22);
     data_set_epsilon(c, P->D, epsilon);
     FHS_HIP(hipStreamSynchronize(c->stream));

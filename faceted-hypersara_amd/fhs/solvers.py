@@ -65,14 +65,18 @@ class Measurement:
             y0 = self._split(Draw, self._forward(Draw, _cube(P.X0)).cpu().numpy().astype(np.complex128))
             del Draw, Graw
             add_noise(P, y0)
-        # Visibilities are reordered inside each block by the grid cell of
-        # their interpolation window (the anchor): neighbouring rows then
-        # gather neighbouring grid points (forward) and the rows of a grid
-        # column sit close together (adjoint).  Per-block results are unchanged.
+        # Visibilities are reordered inside each block by the 32 x 32 grid
+        # tile of their interpolation window's anchor, then by the anchor
+        # inside the tile: the rows of a wavefront then gather one small grid
+        # patch (forward) and the rows a tile's adjoint reads sit together.
+        # Per-block results are unchanged.
+        ty, tx = min(32, Kty), min(32, Ktx)
         self.order = []
         for l in range(P.L):
             if sort_rows:
-                key = coef[l][0]
+                a = coef[l][0].astype(np.int64)
+                ay, ax = a % Kty, a // Kty
+                key = ((ax // tx) * (Kty // ty) + ay // ty) * (ty * tx) + (ax % tx) * ty + ay % ty
                 b = np.concatenate([[0], np.cumsum(P.blocks[l])])
                 idx = np.concatenate([b[i] + np.argsort(key[b[i]:b[i + 1]], kind='stable')
                                       for i in range(len(P.blocks[l]))])
@@ -83,6 +87,7 @@ class Measurement:
         self.v = [P.v[l][self.order[l]] for l in range(P.L)]
         self.y_host = [np.asarray(P.y[l])[self.order[l]] for l in range(P.L)]
         self.G = []
+        self.anchor = [coef[l][0][self.order[l]] for l in range(P.L)]
         for l, (a, cy, cx) in enumerate(coef):
             o = self.order[l]
             self.G.append(GMatrix.from_separable(ctx, a[o], cy[o], cx[o] * np.float32(1.0 / P.sigma_noise[l]),
@@ -266,8 +271,13 @@ def snr_db(X0, x):
 
 
 def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, graph=True,
-           max_total_iter=None, time_limit=None, concurrent=True):
-    """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'."""
+           max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True):
+    """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'.
+
+    With a nonzero ``x_init`` the reference derives the first weights from it
+    (``weights_from_init=True``, ``src/fhs/fhs_initialize_dual_and_weights.m``);
+    ``weights_from_init=False`` keeps the zero start's unit weights, so only
+    the starting point of the same sequence of problems changes."""
     prm = dict(DEFAULT_PARAM)
     prm.update(param or {})
     ctx = meas.ctx
@@ -320,7 +330,7 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     # initial weights (hs_initialize_*, fhs_initialize_dual_and_weights.m:88-112)
     Q = lowrank.Q if lowrank else 0
     w0 = np.ones((Q, L))
-    if xi is not None and bool(torch.any(xi != 0)):
+    if weights_from_init and xi is not None and bool(torch.any(xi != 0)):
         _reweight(eng, alpha, sig, sig_bar)
         w0 = eng._w0
     thresh0 = None if lowrank is None else (beta0[:, None] * w0)

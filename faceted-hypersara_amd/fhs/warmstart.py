@@ -15,9 +15,11 @@ variable from zero (``src/fhs/fhs_initialize_dual_and_weights.m``,
 The network is trained on (dirty, converged) pairs that the plain solver
 generates itself (``make_pairs``), with this project's own forward,
 backward and Adam kernels.  The optimisation problem is not changed: the
-solver run from x_init is the reference iteration with a different starting
-point, and the zero start stays the fallback (``WarmStart.predict`` is never
-required; ``load`` raises on a missing or mismatched file).
+warm-started run keeps the zero start's unit weights (the reference would
+derive them from x_init, which lets a poor prediction move the reweighted
+fixed point), so it is the plain iteration from another starting point, and
+the zero start stays the fallback (``WarmStart.predict`` is never required;
+``load`` raises on a missing or mismatched file).
 """
 import time
 
@@ -34,9 +36,13 @@ LEVELS_DIV = 8          # three 2x2 poolings: image sides padded to a multiple o
 class WarmStart:
     """A U-Net over ``channels``-channel groups of an (L, Nx, Ny) dirty cube."""
 
-    def __init__(self, ctx, channels=8, widths=(16, 32, 64, 128), seed=0):
+    def __init__(self, ctx, channels=8, widths=(16, 32, 64, 128), seed=0, gain=1.0):
         self.ctx, self.channels, self.widths = ctx, channels, tuple(widths)
         self.net = UNet(ctx, cin=channels, cout=channels, widths=widths, seed=seed)
+        # the network sees d / s (s = max of the group's dirty cube) and emits
+        # x * gain / s: a dirty cube integrates the PSF over extended emission
+        # and peaks well above the sky, so gain brings the targets to O(1)
+        self.gain = float(gain)
         self.history = []
 
     # ---------------------------------------------------------- layout ----
@@ -65,7 +71,7 @@ class WarmStart:
 
     def _from_net(self, out, scale, ng, H, W, L, Nx, Ny):
         o = out.reshape(self.channels, ng, H, W).permute(1, 0, 2, 3)[:, :, :Nx, :Ny]
-        o = o * scale[:, None, None, None]
+        o = o * (scale / self.gain)[:, None, None, None]
         return o.reshape(ng * self.channels, Nx, Ny)[:L].clamp_min(0.0).contiguous()
 
     # --------------------------------------------------------- inference --
@@ -79,15 +85,18 @@ class WarmStart:
 
     # ---------------------------------------------------------- training --
     def train(self, pairs, epochs=40, batch=4, lr=1e-3, lr_final=1e-4, augment=True, seed=0, log=None):
-        """Adam on the mean squared error between U(d/s) and x/s over the
-        (dirty, target) pairs ((L, Nx, Ny) GPU tensors, L = channels).
-        Returns the per-epoch mean losses."""
+        """Adam on the mean squared error between U(d/s) and gain x/s over
+        the (dirty, target) pairs ((L, Nx, Ny) GPU tensors, L = channels).
+        On the first call gain is set to the median of max(d)/max(x) over the
+        pairs.  Returns the per-epoch mean losses."""
         rng = np.random.default_rng(seed)
-        n = len(pairs)
+        if not self.history:
+            ratios = [d.amax().item() / max(x.amax().item(), 1e-30) for d, x in pairs]
+            self.gain = float(np.median(ratios))
         xs, ts = [], []
         for d, x in pairs:
             xin, sc, ng, H, W = self._to_net(d.float())
-            tin, _, _, _, _ = self._to_net(x.float(), sc)
+            tin, _, _, _, _ = self._to_net(x.float(), sc / self.gain)
             xs.append(xin.reshape(self.channels, ng, H, W))
             ts.append(tin.reshape(self.channels, ng, H, W))
         X = torch.cat(xs, dim=1)          # (C, n*ng, H, W)
@@ -131,7 +140,7 @@ class WarmStart:
 
     # ------------------------------------------------------- persistence --
     def state(self):
-        out = {'channels': np.array(self.channels), 'widths': np.array(self.widths)}
+        out = {'channels': np.array(self.channels), 'widths': np.array(self.widths), 'gain': np.array(self.gain)}
         for i, layer in enumerate(self.net.layers()):
             out[f'W{i}'] = layer.W.cpu().numpy()
             out[f'b{i}'] = layer.b.cpu().numpy()
@@ -143,7 +152,8 @@ class WarmStart:
     @classmethod
     def load(cls, ctx, path):
         with np.load(path, allow_pickle=False) as z:
-            ws = cls(ctx, channels=int(z['channels']), widths=tuple(int(w) for w in z['widths']))
+            ws = cls(ctx, channels=int(z['channels']), widths=tuple(int(w) for w in z['widths']),
+                     gain=float(z['gain']))
             layers = ws.net.layers()
             for i, layer in enumerate(layers):
                 W, b = z[f'W{i}'], z[f'b{i}']
@@ -210,7 +220,8 @@ def compare_starts(meas, ws, param=None, kind='hs', tol_db=0.1, plain=None, max_
     def stop(t, eng, info):
         return not (info['snr'][-1] >= target)
 
-    _, warm = _solve(kind, meas, prm, x_init=x0, X0=X0, callback=stop, max_total_iter=max_total_iter)
+    _, warm = _solve(kind, meas, prm, x_init=x0, X0=X0, callback=stop, max_total_iter=max_total_iter,
+                     weights_from_init=False)
     ip, iw = iterations_to(plain['snr'], target), iterations_to(warm['snr'], target)
     return {
         'ref_snr': ref, 'target_snr': target, 'x_init_snr': x0_snr,
@@ -218,3 +229,48 @@ def compare_starts(meas, ws, param=None, kind='hs', tol_db=0.1, plain=None, max_
         'warm_iterations': iw, 'warm_s': (warm['time'][iw - 1] + t_pred) if iw else None,
         'predict_s': t_pred, 'plain_total_iterations': plain['iterations'],
     }
+
+
+def default_weights_path():
+    """The network shipped with the package (trained by ``main`` below)."""
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data', 'warmstart_hs256x8.npz')
+
+
+def main(argv=None):
+    """Train the shipped warm-start network on the GPU:
+    ``python -m fhs.warmstart --cubes 200 --out fhs/data/warmstart_hs256x8.npz``.
+    Pairs: seeded HyperSARA cubes of the 512x512x16 workload's statistics at
+    256 x 256 x 8 (65,536 base visibilities, i.e. the same visibility density
+    per grid cell, 10-30 % flags, 4 blocks, 40 dB), converged by the plain
+    solver with 3 reweighting steps."""
+    import argparse
+    from .core import default_context
+    ap = argparse.ArgumentParser(prog='python -m fhs.warmstart')
+    ap.add_argument('--cubes', type=int, default=200)
+    ap.add_argument('--first-seed', type=int, default=1000)
+    ap.add_argument('--size', type=int, default=256)
+    ap.add_argument('--vis', type=int, default=65536)
+    ap.add_argument('--epochs', type=int, default=150)
+    ap.add_argument('--batch', type=int, default=4)
+    ap.add_argument('--lr', type=float, default=1e-3)
+    ap.add_argument('--out', default=default_weights_path())
+    a = ap.parse_args(argv)
+    ctx = default_context()
+    t0 = time.perf_counter()
+    pairs, recs = make_pairs(ctx, range(a.first_seed, a.first_seed + a.cubes), Ny=a.size, Nx=a.size, L=8,
+                             M_base=a.vis, param={'reweighting_max_iter': 3})
+    print(f'{len(pairs)} pairs in {time.perf_counter() - t0:.1f} s '
+          f'(mean {np.mean([r["iterations"] for r in recs]):.0f} PDFB iterations per cube)', flush=True)
+    ws = WarmStart(ctx)
+    t0 = time.perf_counter()
+    losses = ws.train(pairs, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=3e-5,
+                      log=lambda m: print(m, flush=True) if m.split('/')[0].endswith('0') else None)
+    torch.cuda.synchronize()
+    print(f'trained in {time.perf_counter() - t0:.1f} s: loss {losses[0]:.3e} -> {losses[-1]:.3e}', flush=True)
+    ws.save(a.out)
+    print(f'saved {a.out}', flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -22,7 +22,8 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
 
 
-@pytest.mark.parametrize('M,N,K', [(16, 16, 16), (37, 129, 70), (128, 4096, 1152), (200, 64, 33)])
+@pytest.mark.parametrize('M,N,K', [(16, 16, 16), (37, 129, 70), (128, 4096, 1152), (200, 64, 33),
+                                   (16, 72, 20000), (40, 300, 8200)])   # the last two split K
 @pytest.mark.parametrize('ta,tb', [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_gemm_bf16(ctx, M, N, K, ta, tb):
     import ctypes as C
@@ -33,12 +34,18 @@ def test_gemm_bf16(ctx, M, N, K, ta, tb):
     B = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
     bias = rng.normal(size=M).astype(np.float32)
     Cd = torch.zeros((M, N), dtype=torch.float32, device='cuda')
-    ctx.check(lib().fhs_nn_gemm(ctx.h, M, N, K, _ptr(dev(A)), M if ta else K, ta, _ptr(dev(B)), K if tb else N, tb,
-                                _ptr(Cd), N, 1.0, 0.0, _ptr(dev(bias)), 0), 'gemm')
+    Ad, Bd, bd = dev(A), dev(B), dev(bias)     # held until the kernel has run
+    ctx.check(lib().fhs_nn_gemm(ctx.h, M, N, K, _ptr(Ad), M if ta else K, ta, _ptr(Bd), K if tb else N, tb,
+                                _ptr(Cd), N, 1.0, 0.0, _ptr(bd), 0), 'gemm')
+    torch.cuda.synchronize()
     opA = bf16(A).T if ta else bf16(A)
     opB = bf16(B).T if tb else bf16(B)
     ref = opA @ opB + bias[:, None]
     assert rel(Cd.cpu().numpy(), ref) < 1e-5
+    # beta = 1 accumulates into C (the weight-gradient use)
+    ctx.check(lib().fhs_nn_gemm(ctx.h, M, N, K, _ptr(Ad), M if ta else K, ta, _ptr(Bd), K if tb else N, tb,
+                                _ptr(Cd), N, 1.0, 1.0, None, 0), 'gemm beta')
+    assert rel(Cd.cpu().numpy(), 2 * (opA @ opB) + bias[:, None]) < 1e-5
 
 
 def test_layer_primitives(ctx):
@@ -48,18 +55,20 @@ def test_layer_primitives(ctx):
     rng = np.random.default_rng(1)
     Cc, NB, H, W = 5, 2, 12, 10
     x = rng.normal(size=(Cc, NB * H * W)).astype(np.float32)
+    xd = dev(x)
     col = torch.empty((Cc * 9, NB * H * W), dtype=torch.float32, device='cuda')
-    ctx.check(lib().fhs_nn_im2col3(ctx.h, _ptr(dev(x)), Cc, NB, H, W, _ptr(col)), 'im2col')
+    ctx.check(lib().fhs_nn_im2col3(ctx.h, _ptr(xd), Cc, NB, H, W, _ptr(col)), 'im2col')
     assert rel(col.cpu().numpy(), im2col3(x.astype(np.float64), NB, H, W)) < 1e-7
     dcol = rng.normal(size=(Cc * 9, NB * H * W)).astype(np.float32)
+    dcold = dev(dcol)
     din = torch.empty((Cc, NB * H * W), dtype=torch.float32, device='cuda')
-    ctx.check(lib().fhs_nn_col2im3(ctx.h, _ptr(dev(dcol)), Cc, NB, H, W, _ptr(din)), 'col2im')
+    ctx.check(lib().fhs_nn_col2im3(ctx.h, _ptr(dcold), Cc, NB, H, W, _ptr(din)), 'col2im')
     assert rel(din.cpu().numpy(), col2im3(dcol.astype(np.float64), Cc, NB, H, W)) < 1e-6
     p = torch.empty((Cc, NB * H * W // 4), dtype=torch.float32, device='cuda')
-    ctx.check(lib().fhs_nn_maxpool2(ctx.h, _ptr(dev(x)), Cc * NB, H, W, _ptr(p)), 'pool')
+    ctx.check(lib().fhs_nn_maxpool2(ctx.h, _ptr(xd), Cc * NB, H, W, _ptr(p)), 'pool')
     assert rel(p.cpu().numpy(), maxpool2(x.astype(np.float64), NB, H, W)[0]) == 0
     u = torch.empty((Cc, NB * H * W * 4), dtype=torch.float32, device='cuda')
-    ctx.check(lib().fhs_nn_upsample2(ctx.h, _ptr(dev(x)), Cc * NB, H, W, _ptr(u)), 'up')
+    ctx.check(lib().fhs_nn_upsample2(ctx.h, _ptr(xd), Cc * NB, H, W, _ptr(u)), 'up')
     assert rel(u.cpu().numpy(), upsample2(x.astype(np.float64), Cc, NB, H, W)) == 0
 
 
