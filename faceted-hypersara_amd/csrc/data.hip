@@ -25,7 +25,9 @@ struct fhs_data {
   int64_t Mtot = 0;
   int maxM = 0;
   int adj_grid = 0;                    // max adjoint units + grid-tail units over CSC channels
-  int tile_grid = 0;                   // max tiles over tile-gridded channels
+  int tile_grid = 0;                   // max tile work items over tile-gridded channels
+  int red_grid = 0;                    // max split tiles over tile-gridded channels
+  size_t tile_lds = 0;                 // LDS bytes of a tile workgroup (max over channels)
   int64_t No = 0;
   const fhs_nufft* A = nullptr;
   std::vector<int64_t> row_off;        // [L+1]
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_tiled(const GDev* 
                                                                     const float2* __restrict__ v2,
                                                                     const float* __restrict__ lam, int64_t No,
                                                                     float2* __restrict__ g2, int L, int units) {
-  __shared__ TileLds t;
+  extern __shared__ __align__(16) unsigned char t[];
   int l, u;
   if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
@@ -286,16 +288,31 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_tiled(const GDev* 
     g_adjoint_tile(g, u, [=](int r) { return v[r]; }, g2 + (size_t)l * No, t);
 }
 
+// sums of the partial tiles of the tiles split over several work items
+__global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_reduce(const GDev* __restrict__ gd, int64_t No,
+                                                                     float2* __restrict__ g2, int L, int units) {
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
+  const GDev g = gd[l];
+  if (!g.tiled || u >= g.nred) return;
+  g_adjoint_tile_reduce(g, u, g2 + (size_t)l * No);
+}
+
 void launch_data_adjoint(fhs_ctx* ctx, const fhs_data* D, const float2* v2, const float* lam, int dr) {
   const int L = D->L;
   if (D->tile_grid > 0) {
     if (lam)
-      k_data_adjoint_tiled<true><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, 0, ctx->stream>>>(
+      k_data_adjoint_tiled<true><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, D->tile_lds, ctx->stream>>>(
           D->gdev, D->d_row_off, v2, lam, D->No, D->g2, L, D->tile_grid);
     else
-      k_data_adjoint_tiled<false><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, 0, ctx->stream>>>(
+      k_data_adjoint_tiled<false><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, D->tile_lds, ctx->stream>>>(
           D->gdev, D->d_row_off, v2, nullptr, D->No, D->g2, L, D->tile_grid);
     check_launch();
+    if (D->red_grid > 0) {
+      k_data_adjoint_reduce<<<xcd_grid(L, D->red_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->No, D->g2, L,
+                                                                                       D->red_grid);
+      check_launch();
+    }
   }
   if (D->adj_grid > 0) {
     k_data_adjoint<<<xcd_grid(L, D->adj_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam, dr,
@@ -468,6 +485,8 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
       D->maxM = std::max<int>(D->maxM, (int)G[l]->M);
       if (G[l]->d.tiled) {
         D->tile_grid = std::max<int>(D->tile_grid, G[l]->d.nunits);
+        D->red_grid = std::max<int>(D->red_grid, G[l]->d.nred);
+        D->tile_lds = std::max(D->tile_lds, tile_lds_bytes(G[l]->d.Ky, G[l]->d.Kx));
       } else {
         const int64_t tail = (D->No - G[l]->ncols + ADJ_COLS - 1) / ADJ_COLS;
         D->adj_grid = std::max<int>(D->adj_grid, G[l]->d.nunits + (int)tail);

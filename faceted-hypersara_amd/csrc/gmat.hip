@@ -57,7 +57,9 @@ std::vector<int> adjoint_units(const std::vector<int>& colptr) {
 
 struct TileLists {
   int tiles_y = 0, tiles_x = 0;
-  std::vector<int> ptr, rows, cell, order;
+  std::vector<int> rows, cell, empty;
+  std::vector<int4> items, red;
+  int slots = 0;
 };
 
 // every row under each tile its Ky x Kx window touches, sorted by local cell
@@ -89,24 +91,42 @@ TileLists tile_lists(int64_t M, const int32_t* anchor, int Kty, int Ktx, int Ky,
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
   for (int64_t m = 0; m < M; ++m)
     visit(anchor[m], [&](int t, int c) { ent[pos[t]++] = {c, (int)m}; });
-  T.ptr.resize(nt + 1);
   T.rows.resize(cnt[nt]);
   T.cell.resize(cnt[nt]);
-  for (int t = 0; t < nt; ++t) {
-    std::sort(ent.begin() + cnt[t], ent.begin() + cnt[t + 1]);   // by cell, then row
-    T.ptr[t] = (int)cnt[t];
-  }
-  T.ptr[nt] = (int)cnt[nt];
+  for (int t = 0; t < nt; ++t) std::sort(ent.begin() + cnt[t], ent.begin() + cnt[t + 1]);   // by cell, then row
   for (size_t e = 0; e < ent.size(); ++e) {
     T.cell[e] = ent[e].first;
     T.rows[e] = ent[e].second;
   }
-  // heaviest tiles are dispatched first so that they do not form the tail
-  T.order.resize(nt);
-  std::iota(T.order.begin(), T.order.end(), 0);
-  std::stable_sort(T.order.begin(), T.order.end(),
+  // work items, largest first: whole tiles of at most TILE_ITEM listed rows,
+  // larger tiles cut into TILE_ITEM pieces (partial slots), empty tiles in groups
+  std::vector<int> order(nt);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return cnt[a + 1] - cnt[a] > cnt[b + 1] - cnt[b]; });
+  for (int t : order) {
+    const int b = (int)cnt[t], e = (int)cnt[t + 1], n = e - b;
+    if (n == 0) {
+      T.empty.push_back(t);
+      continue;
+    }
+    const int k = (n + TILE_ITEM - 1) / TILE_ITEM;
+    if (k == 1) {
+      T.items.push_back(make_int4(t, b, e, -1));
+      continue;
+    }
+    T.red.push_back(make_int4(t, T.slots, k, 0));
+    for (int j = 0; j < k; ++j)
+      T.items.push_back(make_int4(t, b + j * TILE_ITEM, std::min(e, b + (j + 1) * TILE_ITEM), T.slots++));
+  }
+  std::stable_sort(T.items.begin(), T.items.end(), [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+  for (size_t e = 0; e < T.empty.size(); e += TILE_EMPTY_GROUP)
+    T.items.push_back(make_int4(-1, (int)e, (int)std::min(T.empty.size(), e + TILE_EMPTY_GROUP), -2));
   return T;
+}
+
+__global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint_reduce(GDev g, float2* __restrict__ out) {
+  g_adjoint_tile_reduce(g, blockIdx.x, out);
 }
 
 __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
@@ -121,14 +141,13 @@ __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
 __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint(GDev g, const float2* __restrict__ y,
                                                                const float* __restrict__ lam,
                                                                float2* __restrict__ out) {
-  __shared__ __align__(16) unsigned char lds[ADJ_LDS_BYTES];
+  extern __shared__ __align__(16) unsigned char lds[];
   auto none = [](int) { return make_float2(0.f, 0.f); };
   auto wl = [&](int r) { return cscale(y[r], lam[r]); };
   auto wp = [&](int r) { return y[r]; };
   if (g.tiled) {
-    TileLds& t = *reinterpret_cast<TileLds*>(lds);
-    if (lam) g_adjoint_tile(g, blockIdx.x, wl, out, t);
-    else g_adjoint_tile(g, blockIdx.x, wp, out, t);
+    if (lam) g_adjoint_tile(g, blockIdx.x, wl, out, lds);
+    else g_adjoint_tile(g, blockIdx.x, wp, out, lds);
   } else {
     float2* prod = reinterpret_cast<float2*>(lds);
     if (lam) g_adjoint_unit(g, blockIdx.x, wl, none, out, prod);
@@ -351,18 +370,23 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
     d.anchor = (const int*)g->mem[0];
     d.vals = (const float2*)g->mem[1];
     if (tiled) {
-      g->mem[7] = upload(tl.ptr);
       g->mem[8] = upload(tl.rows);
       g->mem[9] = upload(tl.cell);
-      g->mem[4] = upload(tl.order);
+      g->mem[4] = upload(tl.items);
+      g->mem[5] = upload(tl.empty);
+      g->mem[7] = upload(tl.red);
+      FHS_HIP(hipMalloc(&g->mem[10], std::max<size_t>(1, (size_t)tl.slots * TILE_Y * TILE_X) * sizeof(float2)));
       d.tiled = 1;
       d.tiles_y = tl.tiles_y;
       d.tiles_x = tl.tiles_x;
-      d.tile_ptr = (const int*)g->mem[7];
       d.tile_rows = (const int*)g->mem[8];
       d.tile_cell = (const int*)g->mem[9];
-      d.tile_order = (const int*)g->mem[4];
-      d.nunits = tl.tiles_y * tl.tiles_x;
+      d.items = (const int4*)g->mem[4];
+      d.empty_tiles = (const int*)g->mem[5];
+      d.red = (const int4*)g->mem[7];
+      d.nred = (int)tl.red.size();
+      d.partial = (float2*)g->mem[10];
+      d.nunits = (int)tl.items.size();
     } else {
       g->mem[2] = upload(colptr);
       g->mem[3] = upload(pk);
@@ -411,9 +435,13 @@ int fhs_apply_adjoint_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* y,
   return guard(ctx, [&] {
     FHS_CHECK(G && Fx && y, "apply_adjoint_operator: null argument");
     Scope sc(ctx, "gmat_adjoint");
-    k_gmat_adjoint<<<G->d.nunits, ADJ_THREADS, 0, ctx->stream>>>(
-        G->d, (const float2*)y, Lambda, (float2*)Fx);
+    const size_t lds = G->d.tiled ? tile_lds_bytes(G->d.Ky, G->d.Kx) : ADJ_CSC_LDS_BYTES;
+    k_gmat_adjoint<<<G->d.nunits, ADJ_THREADS, lds, ctx->stream>>>(G->d, (const float2*)y, Lambda, (float2*)Fx);
     check_launch();
+    if (G->d.tiled && G->d.nred > 0) {
+      k_gmat_adjoint_reduce<<<G->d.nred, ADJ_THREADS, 0, ctx->stream>>>(G->d, (float2*)Fx);
+      check_launch();
+    }
   });
 }
 

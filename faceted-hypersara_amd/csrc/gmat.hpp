@@ -43,12 +43,19 @@ struct GDev {
   const int* ucol;      // adjoint work units: unit u = columns [ucol[u], ucol[u+1])
   int nunits;
   const unsigned* csc_pk;  // SEP: (row << 6) | (jx*Ky + jy) per nonzero
-  // SEP on a tiled grid: adjoint by tile gridding (no CSC; nunits = tiles)
+  // SEP on a tiled grid: adjoint by tile gridding (no CSC; nunits = items)
   int tiled, tiles_y, tiles_x;
-  const int* tile_ptr;     // [tiles + 1]
   const int* tile_rows;    // rows listed under every tile their window touches
   const int* tile_cell;    // local anchor cell of each listed row
-  const int* tile_order;   // tiles by decreasing row count (longest first)
+  // work items {tile, begin, end, slot}, largest first: slot -1 writes the
+  // tile, slot >= 0 writes partial[slot] (a tile split over several items,
+  // summed in slot order by the reduce pass), slot -2 zeroes the empty tiles
+  // empty_tiles[begin, end)
+  const int4* items;
+  const int* empty_tiles;
+  const int4* red;         // {tile, first slot, slot count} of the split tiles
+  int nred;
+  float2* partial;         // [slots][TILE_Y * TILE_X]
 };
 
 // the 16 complex slots of separable row m (8 x 16-byte loads)
@@ -102,24 +109,27 @@ __device__ __forceinline__ float2 g_row_apply(const GDev& g, int m, F&& f) {
     const int ky0 = a % g.Kty, kx0 = a / g.Kty;
     float2 cy[SEP_KMAX], cx[SEP_KMAX];
     sep_load(g, m, cy, cx);
-    for (int jx = 0; jx < g.Kx; ++jx) {
-      int kx = kx0 + jx;
-      if (kx >= g.Ktx) kx -= g.Ktx;
-      const int colbase = kx * g.Kty;
-      float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int jy = 0; jy < SEP_KMAX; ++jy) {
-        if (jy < g.Ky) {
-          int ky = ky0 + jy;
-          if (ky >= g.Kty) ky -= g.Kty;
-          const float2 x = f(colbase + ky);
-          s.x += cy[jy].x * x.x - cy[jy].y * x.y;
-          s.y += cy[jy].x * x.y + cy[jy].y * x.x;
+    for (int jx = 0; jx < SEP_KMAX; ++jx) {   // fully unrolled: cy / cx stay in registers
+      if (jx < g.Kx) {
+        int kx = kx0 + jx;
+        if (kx >= g.Ktx) kx -= g.Ktx;
+        const int colbase = kx * g.Kty;
+        float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int jy = 0; jy < SEP_KMAX; ++jy) {
+          if (jy < g.Ky) {
+            int ky = ky0 + jy;
+            if (ky >= g.Kty) ky -= g.Kty;
+            const float2 x = f(colbase + ky);
+            s.x += cy[jy].x * x.x - cy[jy].y * x.y;
+            s.y += cy[jy].x * x.y + cy[jy].y * x.x;
+          }
         }
+        const float2 c = cx[jx];
+        acc.x += c.x * s.x - c.y * s.y;
+        acc.y += c.x * s.y + c.y * s.x;
       }
-      const float2 c = cx[jx];
-      acc.x += c.x * s.x - c.y * s.y;
-      acc.y += c.x * s.y + c.y * s.x;
     }
   } else {
     const int a = g.anchor[m];
@@ -205,33 +215,70 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
 // consecutive cells of each of the Kx cell columns that reach the point.
 // Each output point is written once by one thread, in a fixed order: no
 // atomics, deterministic, no per-nonzero index stream (the CSC is not built).
+// The uv density of a tile varies by orders of magnitude (empty corners, a
+// dense centre), so the work is cut into items of at most TILE_ITEM listed
+// rows: a dense tile is spread over several workgroups whose partial tiles
+// a second pass adds in a fixed order, and empty tiles are zeroed in groups.
 constexpr int TILE_Y = 32, TILE_X = 32;
 constexpr int TILE_CH = ADJ_THREADS;                 // rows staged per chunk
 constexpr int TILE_PTS = TILE_Y * TILE_X / ADJ_THREADS;
 constexpr int TILE_CELLS_MAX = (TILE_Y + SEP_KMAX - 1) * (TILE_X + SEP_KMAX - 1);
+constexpr int TILE_ITEM = 4 * TILE_CH;               // listed rows per work item
+constexpr int TILE_EMPTY_GROUP = 8;                  // empty tiles zeroed per work item
 
-struct TileLds {
-  float2 Q[TILE_CH][SEP_KMAX];
-  float2 C[TILE_CH][SEP_KMAX];
-  int cell[TILE_CH];
-  int start[TILE_CELLS_MAX + 1];
+// LDS of a tile workgroup (dynamic shared memory): the staged rows with an
+// odd stride of Kx + Ky + 1 complex values (conj(cx) w, then conj(cy)), so
+// lanes reading consecutive rows hit different banks; their local cells; the
+// cell start offsets.
+__host__ __device__ inline size_t tile_lds_bytes(int Ky, int Kx) {
+  return (size_t)TILE_CH * (Ky + Kx + 1) * 8 + (size_t)TILE_CH * 4 +
+         ((size_t)(TILE_Y + Ky - 1) * (TILE_X + Kx - 1) + 1) * 4;
+}
+
+struct TileView {
+  float2* qc;
+  int* cell;
+  int* start;
+  int stride;
 };
 
-// LDS of an adjoint workgroup: the CSC unit stage or the tile stage
-constexpr size_t ADJ_LDS_BYTES =
-    sizeof(TileLds) > ADJ_LDS * sizeof(float2) ? sizeof(TileLds) : ADJ_LDS * sizeof(float2);
+__device__ __forceinline__ TileView tile_view(const GDev& g, unsigned char* lds) {
+  TileView v;
+  v.stride = g.Ky + g.Kx + 1;
+  v.qc = reinterpret_cast<float2*>(lds);
+  v.cell = reinterpret_cast<int*>(lds + (size_t)TILE_CH * v.stride * 8);
+  v.start = v.cell + TILE_CH;
+  return v;
+}
+
+// LDS of an adjoint workgroup that may run the CSC unit path (bytes)
+constexpr size_t ADJ_CSC_LDS_BYTES = ADJ_LDS * sizeof(float2);
 
 inline bool tile_grid_ok(int Kty, int Ktx, int Ky, int Kx) {
   return Kty % TILE_Y == 0 && Ktx % TILE_X == 0 && Kty >= TILE_Y + Ky - 1 && Ktx >= TILE_X + Kx - 1;
 }
 
 template <class W>
-__device__ __forceinline__ void g_adjoint_tile(const GDev& g, int t, W&& w, float2* __restrict__ out,
-                                               TileLds& s) {
-  t = g.tile_order[t];
+__device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, float2* __restrict__ out,
+                                               unsigned char* lds) {
+  const TileView s = tile_view(g, lds);
+  const int4 it = g.items[item];
+  if (it.w == -2) {   // a group of empty tiles: zeros
+    for (int e = it.y; e < it.z; ++e) {
+      const int t = g.empty_tiles[e];
+      const int ty = t % g.tiles_y, tx = t / g.tiles_y;
+#pragma unroll
+      for (int q = 0; q < TILE_PTS; ++q) {
+        const int p = threadIdx.x + q * ADJ_THREADS;
+        out[(size_t)(tx * TILE_X + p / TILE_Y) * g.Kty + ty * TILE_Y + p % TILE_Y] = make_float2(0.f, 0.f);
+      }
+    }
+    return;
+  }
+  const int t = it.x;
   const int ty = t % g.tiles_y, tx = t / g.tiles_y;
   const int ncy = TILE_Y + g.Ky - 1, ncx = TILE_X + g.Kx - 1, ncells = ncy * ncx;
-  const int p0 = g.tile_ptr[t], p1 = g.tile_ptr[t + 1];
+  const int p0 = it.y, p1 = it.z;
   float2 acc[TILE_PTS];
 #pragma unroll
   for (int i = 0; i < TILE_PTS; ++i) acc[i] = make_float2(0.f, 0.f);
@@ -243,11 +290,14 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int t, W&& w, floa
       const float2 v = w(r);
       float2 cy[SEP_KMAX], cx[SEP_KMAX];
       sep_load(g, r, cy, cx);
+      float2* q = s.qc + (size_t)i * s.stride;
 #pragma unroll
       for (int j = 0; j < SEP_KMAX; ++j) {
-        const float2 c = cx[j];
-        s.Q[i][j] = make_float2(c.x * v.x + c.y * v.y, c.x * v.y - c.y * v.x);   // conj(c) v
-        s.C[i][j] = make_float2(cy[j].x, -cy[j].y);
+        if (j < g.Kx) {
+          const float2 c = cx[j];
+          q[j] = make_float2(c.x * v.x + c.y * v.y, c.x * v.y - c.y * v.x);   // conj(c) v
+        }
+        if (j < g.Ky) q[g.Kx + j] = make_float2(cy[j].x, -cy[j].y);
       }
       s.cell[i] = g.tile_cell[c0 + i];
     }
@@ -271,7 +321,8 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int t, W&& w, floa
         const int k0 = s.start[base + ly], k1 = s.start[base + ly + g.Ky];
         for (int k = k0; k < k1; ++k) {
           const int jy = ly + g.Ky - 1 - (s.cell[k] - base);
-          const float2 cy = s.C[k][jy], qx = s.Q[k][jx];
+          const float2* q = s.qc + (size_t)k * s.stride;
+          const float2 cy = q[g.Kx + jy], qx = q[jx];
           a.x += cy.x * qx.x - cy.y * qx.y;
           a.y += cy.x * qx.y + cy.y * qx.x;
         }
@@ -279,6 +330,12 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int t, W&& w, floa
       acc[q] = a;
     }
     __syncthreads();
+  }
+  if (it.w >= 0) {   // one of several items of this tile: partial sums
+    float2* part = g.partial + (size_t)it.w * (TILE_Y * TILE_X);
+#pragma unroll
+    for (int q = 0; q < TILE_PTS; ++q) part[threadIdx.x + q * ADJ_THREADS] = acc[q];
+    return;
   }
 #pragma unroll
   for (int q = 0; q < TILE_PTS; ++q) {
@@ -288,11 +345,23 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int t, W&& w, floa
   }
 }
 
+// sum of the partial tiles of split tile r, in slot order
+__device__ __forceinline__ void g_adjoint_tile_reduce(const GDev& g, int r, float2* __restrict__ out) {
+  const int4 rd = g.red[r];
+  const int t = rd.x, ty = t % g.tiles_y, tx = t / g.tiles_y;
+#pragma unroll
+  for (int q = 0; q < TILE_PTS; ++q) {
+    const int p = threadIdx.x + q * ADJ_THREADS;
+    float2 a = make_float2(0.f, 0.f);
+    for (int j = 0; j < rd.z; ++j) a = cadd(a, g.partial[(size_t)(rd.y + j) * (TILE_Y * TILE_X) + p]);
+    out[(size_t)(tx * TILE_X + p / TILE_Y) * g.Kty + ty * TILE_Y + p % TILE_Y] = a;
+  }
+}
 }  // namespace fhs
 
 struct fhs_gmat {
   int device = 0;
   fhs::GDev d{};
   int64_t M = 0, ncols = 0, nnz = 0;
-  void* mem[10] = {};
+  void* mem[12] = {};
 };

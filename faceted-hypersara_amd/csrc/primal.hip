@@ -17,6 +17,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_sqdiff_partial(const float2* __
                                                                 int64_t n, double* partial) {
   __shared__ double sh[RED_THREADS / 64];
   double acc[1] = {0.0};
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const float2 d = csub(x[i], y[i]);
@@ -37,6 +38,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_proj_apply(const float2* __rest
   block_sum<1>(s, sh);
   const double nrm = sqrt(s[0]);
   const float scale = nrm > 0.0 ? (float)fmin(eps / nrm, 1.0) : 1.f;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const float2 yy = y[i];
@@ -53,6 +55,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_primal(float* __restrict__ x,
                                                         double* partial) {
   __shared__ double sh[2 * RED_THREADS / 64];
   double acc[2] = {0.0, 0.0};
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const float prev = x[i];

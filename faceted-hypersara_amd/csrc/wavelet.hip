@@ -114,20 +114,33 @@ __device__ void an_tile(const float* win, int WH, int o, int lf, const float* lo
   __syncthreads();
 }
 
-// load window [wy0, wy0+WH) x [wx0, wx0+WW) of a column-major ny x nx image
+constexpr int WIN_MAX = 2 * WT + WMAX_LF - 2;   // 78
+constexpr int WIN_PER_THREAD = (WIN_MAX * WIN_MAX + WTHREADS - 1) / WTHREADS;   // 24
+
+// load window [wy0, wy0+WH) x [wx0, wx0+WW) of a column-major ny x nx image;
+// every load of a thread is issued before the first LDS store, so the
+// window arrives in one memory latency instead of one per element
 __device__ void load_window(float* win, const float* img, int ny, int nx, int wy0, int wx0, int WH,
                             int WW, float scale) {
-  for (int t = threadIdx.x; t < WH * WW; t += WTHREADS) {
-    const int r = t % WH, c = t / WH;
-    const int gy = wy0 + r, gx = wx0 + c;
-    float v = 0.f;
-    if (gy >= 0 && gy < ny && gx >= 0 && gx < nx) v = img[gy + (int64_t)ny * gx] * scale;
-    win[t] = v;
+  float v[WIN_PER_THREAD];
+  const int n = WH * WW;
+#pragma unroll
+  for (int i = 0; i < WIN_PER_THREAD; ++i) {
+    const int t = threadIdx.x + i * WTHREADS;
+    v[i] = 0.f;
+    if (t < n) {
+      const int r = t % WH, c = t / WH;
+      const int gy = wy0 + r, gx = wx0 + c;
+      if (gy >= 0 && gy < ny && gx >= 0 && gx < nx) v[i] = img[gy + (int64_t)ny * gx];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < WIN_PER_THREAD; ++i) {
+    const int t = threadIdx.x + i * WTHREADS;
+    if (t < n) win[t] = v[i] * scale;
   }
   __syncthreads();
 }
-
-constexpr int WIN_MAX = 2 * WT + WMAX_LF - 2;   // 78
 
 __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const float* __restrict__ x,
                                                     float* __restrict__ v1, float* __restrict__ scr) {
@@ -181,16 +194,27 @@ __device__ void syn_tile(const SynIn& in, int lf, const float* lo, const float* 
   const int cx0 = (ix0 - 1) >= 0 ? (ix0) / 2 : 0;
   const int CN = WT / 2 + lf / 2 + 1;
   const int64_t sz = (int64_t)in.ny * in.nx;
-  // load 4 subband tiles CN x CN: cs[s][cx][cy]
-  for (int t = threadIdx.x; t < 4 * CN * CN; t += WTHREADS) {
-    const int cyl = t % CN, cxl = (t / CN) % CN, s = t / (CN * CN);
-    const int cy = cy0 + cyl, cx = cx0 + cxl;
-    float v = 0.f;
-    if (cy < in.ny && cx < in.nx) {
-      const int64_t p = cy + (int64_t)in.ny * cx;
-      v = s == 0 ? in.a[p] : in.h[(s - 1) * sz + p];
+  // load 4 subband tiles CN x CN: cs[s][cx][cy] (all loads issued first)
+  constexpr int CS_PER_THREAD = (4 * CMAX * CMAX + WTHREADS - 1) / WTHREADS;   // 10
+  float cv[CS_PER_THREAD];
+  const int ncs = 4 * CN * CN;
+#pragma unroll
+  for (int i = 0; i < CS_PER_THREAD; ++i) {
+    const int t = threadIdx.x + i * WTHREADS;
+    cv[i] = 0.f;
+    if (t < ncs) {
+      const int cyl = t % CN, cxl = (t / CN) % CN, s = t / (CN * CN);
+      const int cy = cy0 + cyl, cx = cx0 + cxl;
+      if (cy < in.ny && cx < in.nx) {
+        const int64_t p = cy + (int64_t)in.ny * cx;
+        cv[i] = s == 0 ? in.a[p] : in.h[(s - 1) * sz + p];
+      }
     }
-    cs[t] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < CS_PER_THREAD; ++i) {
+    const int t = threadIdx.x + i * WTHREADS;
+    if (t < ncs) cs[t] = cv[i];
   }
   __syncthreads();
   // y pass: Ly[cx][iy] = sum_k lo[.] a[k][cx] + hi[.] h[k][cx];  Hy: v, d
@@ -325,6 +349,7 @@ __global__ __launch_bounds__(256) void k_l21(SaraGeom G, float* __restrict__ v1,
        c += (int64_t)gridDim.x * blockDim.x) {
     const bool dir = G.dirac && c >= dstart;
     double ss = 0.0;
+#pragma unroll 8
     for (int l = 0; l < L; ++l) {
       float r = v1[(int64_t)l * s + c];
       if (dir && xhat) r += xhat[(int64_t)l * G.Ny * G.Nx + (c - dstart)] * G.norm;
@@ -341,6 +366,7 @@ __global__ __launch_bounds__(256) void k_l21(SaraGeom G, float* __restrict__ v1,
     }
     const double t = beta * (w ? (double)w[c] : 1.0);
     const float f = (float)(fmax(l2 - t, 0.0) / (l2 + 2.220446049250313e-16));
+#pragma unroll 8
     for (int l = 0; l < L; ++l) {
       float r = v1[(int64_t)l * s + c];
       if (dir && xhat) r += xhat[(int64_t)l * G.Ny * G.Nx + (c - dstart)] * G.norm;
