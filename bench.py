@@ -47,6 +47,9 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-solve', action='store_true', help='skip the time-to-SNR solve')
     ap.add_argument('--solve-seconds', type=float, default=60.0)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
+                         'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
     return ap.parse_args()
 
@@ -98,6 +101,15 @@ def main():
     import fhs
     from fhs import solvers, synth
 
+    if args.copy_calibration:
+        n = 256 << 20
+        a = torch.ones(n // 4, dtype=torch.float32, device='cuda')
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': n, 'copy_bytes_written': n, 'repeats': 3}))
+        return
     ctx = fhs.default_context(0)
     t_setup = time.perf_counter()
     prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
@@ -140,6 +152,13 @@ def main():
     dominant = max((k for k in rep if k in alg), key=lambda k: rep[k][0])
     avg_ms = rep[dominant][0] / rep[dominant][1]
     achieved = alg[dominant] / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE passes
+    # (copy-calibrated) for this workload, when that summary exists
+    traffic = None
+    tpath = os.path.join(ROOT, 'profiles', 'hs512x16_pmc_traffic.json')
+    if (args.size, args.channels, args.vis, args.seed) == (512, 16, 200000, 0) and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get('traffic_bytes_per_launch', {}).get(dominant)
     kernels = {k: {'ms_per_launch': rep[k][0] / rep[k][1], 'launches_per_iter': rep[k][1] / n_timed,
                    'GBps': (alg[k] / (rep[k][0] / rep[k][1] * 1e-3) / 1e9) if k in alg else None}
                for k in rep}
@@ -240,7 +259,7 @@ def main():
                    'setup_s': round(setup_s, 2)},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1),
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
-                     'traffic': None, 'avg_launch_ms': round(avg_ms, 5),
+                     'traffic': traffic, 'avg_launch_ms': round(avg_ms, 5),
                      'algorithmic_bytes_per_launch': int(alg[dominant]),
                      'survey_formula_bytes_per_launch': int(sum(8 * g.nnz + 12 * g.M + 8 * meas.nufft.No
                                                                 for g in meas.G))},

@@ -210,7 +210,8 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
 // ("local cell": (TILE_Y + Ky - 1) x (TILE_X + Kx - 1) cells, x-major).  One
 // workgroup owns one tile: it stages 256 listed rows at a time in LDS as
 //   Q[k][jx] = conj(cx[jx]) w(row),   C[k][jy] = conj(cy[jy]),
-// finds where each local cell starts in the staged chunk (binary search), and
+// finds where each local cell starts in the staged chunk (the first row of
+// each occupied cell marks it, a block-wide suffix minimum fills the rest), and
 // every thread then sums, for the grid points it owns, the rows of the Ky
 // consecutive cells of each of the Kx cell columns that reach the point.
 // Each output point is written once by one thread, in a fixed order: no
@@ -231,7 +232,7 @@ constexpr int TILE_EMPTY_GROUP = 8;                  // empty tiles zeroed per w
 // lanes reading consecutive rows hit different banks; their local cells; the
 // cell start offsets.
 __host__ __device__ inline size_t tile_lds_bytes(int Ky, int Kx) {
-  return (size_t)TILE_CH * (Ky + Kx + 1) * 8 + (size_t)TILE_CH * 4 +
+  return (size_t)TILE_CH * (Ky + Kx + 1) * 8 + (size_t)(TILE_CH + ADJ_THREADS / 64) * 4 +
          ((size_t)(TILE_Y + Ky - 1) * (TILE_X + Kx - 1) + 1) * 4;
 }
 
@@ -247,7 +248,7 @@ __device__ __forceinline__ TileView tile_view(const GDev& g, unsigned char* lds)
   v.stride = g.Ky + g.Kx + 1;
   v.qc = reinterpret_cast<float2*>(lds);
   v.cell = reinterpret_cast<int*>(lds + (size_t)TILE_CH * v.stride * 8);
-  v.start = v.cell + TILE_CH;
+  v.start = v.cell + TILE_CH + ADJ_THREADS / 64;   // cell[TILE_CH], scan scratch, start
   return v;
 }
 
@@ -256,6 +257,40 @@ constexpr size_t ADJ_CSC_LDS_BYTES = ADJ_LDS * sizeof(float2);
 
 inline bool tile_grid_ok(int Kty, int Ktx, int Ky, int Kx) {
   return Kty % TILE_Y == 0 && Ktx % TILE_X == 0 && Kty >= TILE_Y + Ky - 1 && Ktx >= TILE_X + Kx - 1;
+}
+
+// a[c] = min(a[c], a[c+1], ..., a[n-1]) for c < n, by the ADJ_THREADS
+// threads of the workgroup (each owns a contiguous segment); wsum: LDS
+// scratch of ADJ_THREADS / 64 ints.  Ends with a barrier.
+__device__ __forceinline__ void tile_suffix_min(int* a, int n, int* wsum) {
+  constexpr int PER = (TILE_CELLS_MAX + 1 + ADJ_THREADS - 1) / ADJ_THREADS;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int v[PER];
+  int m = 0x7fffffff;
+#pragma unroll
+  for (int j = PER - 1; j >= 0; --j) {
+    const int idx = t * PER + j;
+    if (idx < n) m = min(m, a[idx]);
+    v[j] = m;
+  }
+  // inclusive suffix minimum of the segment minima over the lanes of the wave
+  int x = m;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_down(x, o, 64);
+    if (lane + o < 64) x = min(x, y);
+  }
+  if (lane == 0) wsum[wv] = x;
+  int ex = __shfl_down(x, 1, 64);
+  if (lane == 63) ex = 0x7fffffff;
+  __syncthreads();
+  for (int u = wv + 1; u < ADJ_THREADS / 64; ++u) ex = min(ex, wsum[u]);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int idx = t * PER + j;
+    if (idx < n) a[idx] = min(v[j], ex);
+  }
+  __syncthreads();
 }
 
 template <class W>
@@ -301,28 +336,28 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
       }
       s.cell[i] = g.tile_cell[c0 + i];
     }
+    for (int c = threadIdx.x; c <= ncells; c += ADJ_THREADS) s.start[c] = n;
     __syncthreads();
-    for (int c = threadIdx.x; c <= ncells; c += ADJ_THREADS) {   // first staged row with cell >= c
-      int lo = 0, hi = n;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (s.cell[mid] < c) lo = mid + 1; else hi = mid;
-      }
-      s.start[c] = lo;
-    }
+    // start[c] = first staged row with cell >= c: the first row of every
+    // occupied cell marks it, then a block-wide suffix minimum fills the rest
+    if (i < n && (i == 0 || s.cell[i - 1] != s.cell[i])) s.start[s.cell[i]] = i;
     __syncthreads();
+    tile_suffix_min(s.start, ncells + 1, s.cell + TILE_CH);
+    // occupied cell columns of this chunk (cells are sorted, x-major)
+    const int cxmin = s.cell[0] / ncy, cxmax = s.cell[n - 1] / ncy;
 #pragma unroll
     for (int q = 0; q < TILE_PTS; ++q) {
       const int p = threadIdx.x + q * ADJ_THREADS;
       const int ly = p % TILE_Y, lx = p / TILE_Y;
       float2 a = acc[q];
-      for (int jx = 0; jx < g.Kx; ++jx) {
+      const int jx0 = max(0, lx + g.Kx - 1 - cxmax), jx1 = min(g.Kx - 1, lx + g.Kx - 1 - cxmin);
+      for (int jx = jx0; jx <= jx1; ++jx) {
         const int base = (lx + g.Kx - 1 - jx) * ncy;
         const int k0 = s.start[base + ly], k1 = s.start[base + ly + g.Ky];
         for (int k = k0; k < k1; ++k) {
           const int jy = ly + g.Ky - 1 - (s.cell[k] - base);
-          const float2* q = s.qc + (size_t)k * s.stride;
-          const float2 cy = q[g.Kx + jy], qx = q[jx];
+          const float2* qq = s.qc + (size_t)k * s.stride;
+          const float2 cy = qq[g.Kx + jy], qx = qq[jx];
           a.x += cy.x * qx.x - cy.y * qx.y;
           a.y += cy.x * qx.y + cy.y * qx.x;
         }
