@@ -28,6 +28,8 @@ struct fhs_data {
   int tile_grid = 0;                   // max tile work items over tile-gridded channels
   int red_grid = 0;                    // max split tiles over tile-gridded channels
   size_t tile_lds = 0;                 // LDS bytes of a tile workgroup (max over channels)
+  int fwd_grid = 0;                    // max forward work items over tile-gridded channels
+  bool any_rows = false;               // some channel without tile gridding (row kernels)
   int64_t No = 0;
   const fhs_nufft* A = nullptr;
   std::vector<int64_t> row_off;        // [L+1]
@@ -79,6 +81,7 @@ __global__ __launch_bounds__(256) void k_data_forward(const GDev* __restrict__ g
   int l, u;
   if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
+  if (g.tiled) return;   // k_data_forward_tiled
   const int m = u * blockDim.x + threadIdx.x;
   if (m >= g.M) return;
   const float2 gx = g_row_dot(g, m, Fx + (size_t)l * No);
@@ -330,6 +333,7 @@ __global__ __launch_bounds__(256) void k_data_apply(const GDev* __restrict__ gd,
   int l, u;
   if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
+  if (g.tiled) return;   // k_data_apply_tiled
   const int m = u * blockDim.x + threadIdx.x;
   if (m >= g.M) return;
   const int64_t i = row_off[l] + m;
@@ -337,6 +341,58 @@ __global__ __launch_bounds__(256) void k_data_apply(const GDev* __restrict__ gd,
   if (lam) v = cscale(v, lam[i]);
   y[i] = v;
 }
+
+// the same two operations for the channels with a tile-gridded separable G:
+// one workgroup per forward work item, grid window staged in LDS
+__global__ __launch_bounds__(ADJ_THREADS) void k_data_forward_tiled(const GDev* __restrict__ gd,
+                                                                    const int64_t* __restrict__ row_off,
+                                                                    const float2* __restrict__ Fx, int64_t No,
+                                                                    float2* __restrict__ gx_old,
+                                                                    float2* __restrict__ r2, int L, int units) {
+  __shared__ float2 win[FWD_WIN_MAX];
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
+  const GDev g = gd[l];
+  if (!g.tiled || u >= g.nfitems) return;
+  float2* __restrict__ go = gx_old + row_off[l];
+  float2* __restrict__ rr = r2 + row_off[l];
+  g_forward_tile(g, u, Fx + (size_t)l * No, win, [=](int r, float2 gx) {
+    const float2 old = go[r];
+    rr[r] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
+    go[r] = gx;
+  });
+}
+
+__global__ __launch_bounds__(ADJ_THREADS) void k_data_apply_tiled(const GDev* __restrict__ gd,
+                                                                  const int64_t* __restrict__ row_off,
+                                                                  const float2* __restrict__ Fx, int64_t No,
+                                                                  const float* __restrict__ lam,
+                                                                  float2* __restrict__ y, int L, int units) {
+  __shared__ float2 win[FWD_WIN_MAX];
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
+  const GDev g = gd[l];
+  if (!g.tiled || u >= g.nfitems) return;
+  float2* __restrict__ yo = y + row_off[l];
+  const float* __restrict__ lm = lam ? lam + row_off[l] : nullptr;
+  g_forward_tile(g, u, Fx + (size_t)l * No, win,
+                 [=](int r, float2 v) { yo[r] = lm ? cscale(v, lm[r]) : v; });
+}
+
+// G Fx -> r2 = 2 G Fx - gx_old, gx_old <- G Fx (non-DR form) for all channels
+void launch_data_forward(fhs_ctx* ctx, const fhs_data* D, float2* gx_old, unsigned row_grid, int row_units) {
+  if (D->fwd_grid > 0) {
+    k_data_forward_tiled<<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
+        D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L, D->fwd_grid);
+    check_launch();
+  }
+  if (D->any_rows) {
+    k_data_forward<<<row_grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L,
+                                                      row_units);
+    check_launch();
+  }
+}
+
 
 __global__ void k_sum_blocks(const double* __restrict__ norm_res, const double* __restrict__ eps,
                              int nb, double* __restrict__ out2) {
@@ -376,7 +432,7 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
       k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2, L,
                                                        units);
     else
-      k_data_forward<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, L, units);
+      launch_data_forward(ctx, D, gx_old, grid, units);
     check_launch();
   }
   {
@@ -418,7 +474,7 @@ void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int
     k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2,
                                                      D->L, units);
   else
-    k_data_forward<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L, units);
+    launch_data_forward(ctx, D, gx_old, grid, units);
   check_launch();
 }
 
@@ -427,8 +483,14 @@ void data_forward(fhs_ctx* ctx, fhs_data* D, const float* x, const float* lam, f
   nufft_forward(ctx, D->A, x, D->L, D->Fx);
   Scope sc(ctx, "data_forward");
   const int units = (int)cdiv(D->maxM, 256);
-  k_data_apply<<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L,
-                                                               units);
+  if (D->fwd_grid > 0) {
+    k_data_apply_tiled<<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
+        D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L, D->fwd_grid);
+    check_launch();
+  }
+  if (D->any_rows)
+    k_data_apply<<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L,
+                                                                 units);
   check_launch();
 }
 
@@ -487,7 +549,9 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
         D->tile_grid = std::max<int>(D->tile_grid, G[l]->d.nunits);
         D->red_grid = std::max<int>(D->red_grid, G[l]->d.nred);
         D->tile_lds = std::max(D->tile_lds, tile_lds_bytes(G[l]->d.Ky, G[l]->d.Kx));
+        D->fwd_grid = std::max<int>(D->fwd_grid, G[l]->d.nfitems);
       } else {
+        D->any_rows = true;
         const int64_t tail = (D->No - G[l]->ncols + ADJ_COLS - 1) / ADJ_COLS;
         D->adj_grid = std::max<int>(D->adj_grid, G[l]->d.nunits + (int)tail);
       }

@@ -60,6 +60,8 @@ struct TileLists {
   std::vector<int> rows, cell, empty;
   std::vector<int4> items, red;
   int slots = 0;
+  std::vector<int> prim_rows;
+  std::vector<int4> fitems;
 };
 
 // every row under each tile its Ky x Kx window touches, sorted by local cell
@@ -122,11 +124,33 @@ TileLists tile_lists(int64_t M, const int32_t* anchor, int Kty, int Ktx, int Ky,
   std::stable_sort(T.items.begin(), T.items.end(), [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
   for (size_t e = 0; e < T.empty.size(); e += TILE_EMPTY_GROUP)
     T.items.push_back(make_int4(-1, (int)e, (int)std::min(T.empty.size(), e + TILE_EMPTY_GROUP), -2));
+  // forward: every row once, under the tile of its anchor, rows ascending
+  std::vector<int64_t> pc(nt + 1, 0);
+  auto prim = [&](int a) { return (a / Kty / TILE_X) * T.tiles_y + (a % Kty) / TILE_Y; };
+  for (int64_t m = 0; m < M; ++m) pc[prim(anchor[m]) + 1]++;
+  for (int t = 0; t < nt; ++t) pc[t + 1] += pc[t];
+  T.prim_rows.resize(M);
+  {
+    std::vector<int64_t> pp(pc.begin(), pc.end() - 1);
+    for (int64_t m = 0; m < M; ++m) T.prim_rows[pp[prim(anchor[m])]++] = (int)m;
+  }
+  for (int t = 0; t < nt; ++t)
+    for (int64_t b = pc[t]; b < pc[t + 1]; b += FWD_ITEM)
+      T.fitems.push_back(make_int4(t, (int)b, (int)std::min<int64_t>(pc[t + 1], b + FWD_ITEM), 0));
+  std::stable_sort(T.fitems.begin(), T.fitems.end(),
+                   [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
   return T;
 }
 
 __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint_reduce(GDev g, float2* __restrict__ out) {
   g_adjoint_tile_reduce(g, blockIdx.x, out);
+}
+
+__global__ __launch_bounds__(ADJ_THREADS) void k_gmat_forward_tiled(GDev g, const float2* __restrict__ Fx,
+                                                                  const float* __restrict__ lam,
+                                                                  float2* __restrict__ y) {
+  __shared__ float2 win[FWD_WIN_MAX];
+  g_forward_tile(g, blockIdx.x, Fx, win, [&](int r, float2 v) { y[r] = lam ? cscale(v, lam[r]) : v; });
 }
 
 __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
@@ -376,6 +400,8 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
       g->mem[5] = upload(tl.empty);
       g->mem[7] = upload(tl.red);
       FHS_HIP(hipMalloc(&g->mem[10], std::max<size_t>(1, (size_t)tl.slots * TILE_Y * TILE_X) * sizeof(float2)));
+      g->mem[11] = upload(tl.prim_rows);
+      g->mem[12] = upload(tl.fitems);
       d.tiled = 1;
       d.tiles_y = tl.tiles_y;
       d.tiles_x = tl.tiles_x;
@@ -386,6 +412,9 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
       d.red = (const int4*)g->mem[7];
       d.nred = (int)tl.red.size();
       d.partial = (float2*)g->mem[10];
+      d.prim_rows = (const int*)g->mem[11];
+      d.fitems = (const int4*)g->mem[12];
+      d.nfitems = (int)tl.fitems.size();
       d.nunits = (int)tl.items.size();
     } else {
       g->mem[2] = upload(colptr);
@@ -423,8 +452,13 @@ int fhs_apply_direct_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* Fx,
   return guard(ctx, [&] {
     FHS_CHECK(G && Fx && y, "apply_direct_operator: null argument");
     Scope sc(ctx, "gmat_forward");
-    k_gmat_forward<<<cdiv(G->M, 256), 256, 0, ctx->stream>>>(
-        G->d, (const float2*)Fx, Lambda, (float2*)y);
+    if (G->d.tiled) {
+      if (G->d.nfitems > 0)
+        k_gmat_forward_tiled<<<G->d.nfitems, ADJ_THREADS, 0, ctx->stream>>>(G->d, (const float2*)Fx, Lambda,
+                                                                           (float2*)y);
+    } else {
+      k_gmat_forward<<<cdiv(G->M, 256), 256, 0, ctx->stream>>>(G->d, (const float2*)Fx, Lambda, (float2*)y);
+    }
     check_launch();
   });
 }

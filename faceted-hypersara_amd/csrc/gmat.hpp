@@ -56,6 +56,11 @@ struct GDev {
   const int4* red;         // {tile, first slot, slot count} of the split tiles
   int nred;
   float2* partial;         // [slots][TILE_Y * TILE_X]
+  // forward by tile: rows grouped by the tile of their anchor (ascending row
+  // order inside a tile), work items {tile, begin, end, 0} of <= FWD_ITEM rows
+  const int* prim_rows;
+  const int4* fitems;
+  int nfitems;
 };
 
 // the 16 complex slots of separable row m (8 x 16-byte loads)
@@ -226,6 +231,8 @@ constexpr int TILE_PTS = TILE_Y * TILE_X / ADJ_THREADS;
 constexpr int TILE_CELLS_MAX = (TILE_Y + SEP_KMAX - 1) * (TILE_X + SEP_KMAX - 1);
 constexpr int TILE_ITEM = 4 * TILE_CH;               // listed rows per work item
 constexpr int TILE_EMPTY_GROUP = 8;                  // empty tiles zeroed per work item
+constexpr int FWD_ITEM = 1024;                       // rows per forward work item
+constexpr int FWD_WIN_MAX = (TILE_Y + SEP_KMAX - 1) * (TILE_X + SEP_KMAX - 1);
 
 // LDS of a tile workgroup (dynamic shared memory): the staged rows with an
 // odd stride of Kx + Ky + 1 complex values (conj(cx) w, then conj(cy)), so
@@ -392,11 +399,71 @@ __device__ __forceinline__ void g_adjoint_tile_reduce(const GDev& g, int r, floa
     out[(size_t)(tx * TILE_X + p / TILE_Y) * g.Kty + ty * TILE_Y + p % TILE_Y] = a;
   }
 }
+// Forward (degridding) of one work item: the (TILE_Y + Ky - 1) x (TILE_X +
+// Kx - 1) grid window of the tile is staged in LDS once (all loads issued
+// before the stores), then every row of the item reads its Ky x Kx taps from
+// LDS instead of gathering them through the caches.  emit(r, G Fx) stores
+// row r's value.
+template <class E>
+__device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const float2* __restrict__ Fx,
+                                               float2* win, E&& emit) {
+  const int4 it = g.fitems[item];
+  const int t = it.x;
+  const int ty0 = (t % g.tiles_y) * TILE_Y, tx0 = (t / g.tiles_y) * TILE_X;
+  const int wy = TILE_Y + g.Ky - 1, wx = TILE_X + g.Kx - 1, nw = wy * wx;
+  constexpr int PER = (FWD_WIN_MAX + ADJ_THREADS - 1) / ADJ_THREADS;
+  float2 v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + i * ADJ_THREADS;
+    v[i] = make_float2(0.f, 0.f);
+    if (e < nw) {
+      int ky = ty0 + e % wy, kx = tx0 + e / wy;
+      if (ky >= g.Kty) ky -= g.Kty;
+      if (kx >= g.Ktx) kx -= g.Ktx;
+      v[i] = Fx[(size_t)kx * g.Kty + ky];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + i * ADJ_THREADS;
+    if (e < nw) win[e] = v[i];
+  }
+  __syncthreads();
+  for (int k = it.y + threadIdx.x; k < it.z; k += ADJ_THREADS) {
+    const int r = g.prim_rows[k];
+    const int a = g.anchor[r];
+    const int ly = a % g.Kty - ty0, lx = a / g.Kty - tx0;
+    float2 cy[SEP_KMAX], cx[SEP_KMAX];
+    sep_load(g, r, cy, cx);
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int jx = 0; jx < SEP_KMAX; ++jx) {
+      if (jx < g.Kx) {
+        const float2* col = win + (lx + jx) * wy + ly;
+        float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int jy = 0; jy < SEP_KMAX; ++jy) {
+          if (jy < g.Ky) {
+            const float2 x = col[jy];
+            s.x += cy[jy].x * x.x - cy[jy].y * x.y;
+            s.y += cy[jy].x * x.y + cy[jy].y * x.x;
+          }
+        }
+        const float2 c = cx[jx];
+        acc.x += c.x * s.x - c.y * s.y;
+        acc.y += c.x * s.y + c.y * s.x;
+      }
+    }
+    emit(r, acc);
+  }
+}
+
 }  // namespace fhs
 
 struct fhs_gmat {
   int device = 0;
   fhs::GDev d{};
   int64_t M = 0, ncols = 0, nnz = 0;
-  void* mem[12] = {};
+  void* mem[14] = {};
 };

@@ -124,8 +124,18 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_fwd_y(const float* __restric
   const int ix = blockIdx.x, l = blockIdx.y;
   const float* xc = x + ((size_t)l * Nx + ix) * Ny;
   const float scx = sx[ix];
-  for (int i = threadIdx.x; i < N; i += NT)
-    s[i] = i < Ny ? make_float2(xc[i] * sy[i] * scx, 0.f) : make_float2(0.f, 0.f);
+  constexpr int PER = (N + NT - 1) / NT;
+  float v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = threadIdx.x + j * NT;
+    v[j] = i < Ny ? xc[i] * sy[i] * scx : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < N) s[i] = make_float2(v[j], 0.f);
+  }
   __syncthreads();
   fft_lds<N, 1, NT, false>(s, tw);
   float2* out = T + ((size_t)l * Nx + ix) * N;
@@ -142,10 +152,16 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restr
   __shared__ float2 s[N * B];
   const int ky0 = blockIdx.x * B, l = blockIdx.y;
   const float2* Tl = T + (size_t)l * Nx * Kty;
-  for (int t = threadIdx.x; t < N * B; t += NT) {
+  constexpr int PER = N * B / NT;   // every load issued before the first LDS store
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
     const int b = t % B, i = t / B;
-    s[t] = i < Nx ? Tl[(size_t)i * Kty + ky0 + b] : make_float2(0.f, 0.f);
+    v[j] = i < Nx ? Tl[(size_t)i * Kty + ky0 + b] : make_float2(0.f, 0.f);
   }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s[threadIdx.x + j * NT] = v[j];
   __syncthreads();
   fft_lds<N, B, NT, false>(s, tw);
   float2* F = Fx + (size_t)l * N * Kty;
@@ -165,10 +181,15 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
   __shared__ float2 s[N * B];
   const int ky0 = blockIdx.x * B, l = blockIdx.y;
   const float2* gl = g + (size_t)l * N * Kty;
-  for (int t = threadIdx.x; t < N * B; t += NT) {
-    const int b = t % B, k = t / B;
-    s[t] = gl[(size_t)k * Kty + ky0 + b];
+  constexpr int PER = N * B / NT;   // every load issued before the first LDS store
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    v[j] = gl[(size_t)(t / B) * Kty + ky0 + t % B];
   }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s[threadIdx.x + j * NT] = v[j];
   __syncthreads();
   fft_lds<N, B, NT, true>(s, tw);
   float2* Tl = T + (size_t)l * Nx * Kty;
@@ -190,7 +211,18 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_adj_y(const float2* __restri
   __shared__ float2 s[N];
   const int ix = blockIdx.x, l = blockIdx.y;
   const float2* Tc = T + ((size_t)l * Nx + ix) * N;
-  for (int i = threadIdx.x; i < N; i += NT) s[i] = Tc[i];
+  constexpr int PER = (N + NT - 1) / NT;
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = threadIdx.x + j * NT;
+    v[j] = i < N ? Tc[i] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < N) s[i] = v[j];
+  }
   __syncthreads();
   fft_lds<N, 1, NT, true>(s, tw);
   const float sc = (scale ? scale[l] : 1.f) * sx[ix];

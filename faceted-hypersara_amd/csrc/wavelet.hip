@@ -67,19 +67,26 @@ struct AnOut {
   bool last;
 };
 
-__device__ void an_tile(const float* win, int WH, int o, int lf, const float* lo, const float* hi,
-                        int ky0, int kx0, float* Lx, float* Hx, const AnOut& out) {
-  const int H = 2 * WT + lf - 2;   // rows of the basis' window
+template <int LF>
+__device__ void an_tile_t(const float* win, int WH, const float* lo, const float* hi, int ky0, int kx0, float* Lx,
+                          float* Hx, const AnOut& out) {
+  constexpr int H = 2 * WT + LF - 2;   // rows of the window
+  float rlo[LF], rhi[LF];
+#pragma unroll
+  for (int j = 0; j < LF; ++j) {
+    rlo[j] = lo[j];
+    rhi[j] = hi[j];
+  }
   // x pass: Lx[kx][r], Hx[kx][r] for r < H, kx < WT
   for (int t = threadIdx.x; t < H * WT; t += WTHREADS) {
     const int r = t % H, kx = t / H;
     float sl = 0.f, sh = 0.f;
-    const float* col = win + (size_t)(o + 2 * kx) * WH + o + r;
-#pragma unroll 4
-    for (int j = 0; j < lf; ++j) {
+    const float* col = win + (size_t)(2 * kx) * WH + r;
+#pragma unroll
+    for (int j = 0; j < LF; ++j) {
       const float w = col[(size_t)j * WH];
-      sl += lo[j] * w;
-      sh += hi[j] * w;
+      sl += rlo[j] * w;
+      sh += rhi[j] * w;
     }
     Lx[kx * H + r] = sl;
     Hx[kx * H + r] = sh;
@@ -93,12 +100,12 @@ __device__ void an_tile(const float* win, int WH, int o, int lf, const float* lo
       const float* lc = Lx + kx * H + 2 * ky;
       const float* hc = Hx + kx * H + 2 * ky;
       float a = 0.f, h = 0.f, v = 0.f, d = 0.f;
-#pragma unroll 4
-      for (int j = 0; j < lf; ++j) {
-        a += lo[j] * lc[j];
-        h += hi[j] * lc[j];
-        v += lo[j] * hc[j];
-        d += hi[j] * hc[j];
+#pragma unroll
+      for (int j = 0; j < LF; ++j) {
+        a += rlo[j] * lc[j];
+        h += rhi[j] * lc[j];
+        v += rlo[j] * hc[j];
+        d += rhi[j] * hc[j];
       }
       const int64_t p = gy + (int64_t)out.ny * gx;
       const int64_t sz = (int64_t)out.ny * out.nx;
@@ -112,6 +119,21 @@ __device__ void an_tile(const float* win, int WH, int o, int lf, const float* lo
     }
   }
   __syncthreads();
+}
+
+// filter lengths of db1..db8 are 2, 4, ..., 16: one unrolled instance each
+__device__ void an_tile(const float* win, int WH, int lf, const float* lo, const float* hi, int ky0, int kx0,
+                        float* Lx, float* Hx, const AnOut& out) {
+  switch (lf) {
+    case 2: an_tile_t<2>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 4: an_tile_t<4>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 6: an_tile_t<6>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 8: an_tile_t<8>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 10: an_tile_t<10>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 12: an_tile_t<12>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 14: an_tile_t<14>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    default: an_tile_t<16>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+  }
 }
 
 constexpr int WIN_MAX = 2 * WT + WMAX_LF - 2;   // 78
@@ -172,7 +194,7 @@ __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const flo
   out.h_off = G.boff[b] + G.doff[b][k];
   out.ny = G.ny[b][k];
   out.nx = G.nx[b][k];
-  an_tile(win, WH, 0, lf, flo, fhi, ky0, kx0, Lx, Hx, out);
+  an_tile(win, WH, lf, flo, fhi, ky0, kx0, Lx, Hx, out);
 }
 
 // ---- synthesis -------------------------------------------------------------
@@ -187,8 +209,101 @@ struct SynIn {
   int ny, nx;
 };
 
-// accumulates the tile (WT*WT/WTHREADS values per thread) into acc
-__device__ void syn_tile(const SynIn& in, int lf, const float* lo, const float* hi, int iy0, int ix0,
+// accumulates the tile (WT*WT/WTHREADS values per thread) into acc.
+// Output i takes the LF/2 coefficients k = floor(i/2) + m (m < LF/2) with
+// filter tap j = (i & 1) + LF - 2 - 2m; the staged tile is zero beyond the
+// subband, so no clamping is needed.
+template <int LF>
+__device__ void syn_tile_t(const SynIn& in, const float* lo, const float* hi, int iy0, int ix0, int py, int px,
+                           float* cs, float* Ly, float* Hy, float* acc) {
+  constexpr int CN = WT / 2 + LF / 2 + 1;
+  const int cy0 = iy0 / 2, cx0 = ix0 / 2;
+  const int64_t sz = (int64_t)in.ny * in.nx;
+  // load 4 subband tiles CN x CN: cs[s][cx][cy] (all loads issued first)
+  constexpr int NCS = 4 * CN * CN;
+  constexpr int CS_PER_THREAD = (NCS + WTHREADS - 1) / WTHREADS;
+  float cv[CS_PER_THREAD];
+#pragma unroll
+  for (int i = 0; i < CS_PER_THREAD; ++i) {
+    const int t = threadIdx.x + i * WTHREADS;
+    cv[i] = 0.f;
+    if (t < NCS) {
+      const int cyl = t % CN, cxl = (t / CN) % CN, s = t / (CN * CN);
+      const int cy = cy0 + cyl, cx = cx0 + cxl;
+      if (cy < in.ny && cx < in.nx) {
+        const int64_t p = cy + (int64_t)in.ny * cx;
+        cv[i] = s == 0 ? in.a[p] : in.h[(s - 1) * sz + p];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < CS_PER_THREAD; ++i) {
+    const int t = threadIdx.x + i * WTHREADS;
+    if (t < NCS) cs[t] = cv[i];
+  }
+  __syncthreads();
+  // the caller's filter writes are visible after the barrier above
+  float rlo[LF], rhi[LF];
+#pragma unroll
+  for (int j = 0; j < LF; ++j) {
+    rlo[j] = lo[j];
+    rhi[j] = hi[j];
+  }
+  // y pass: Ly[cx][iy] = sum_k lo a[k][cx] + hi h[k][cx];  Hy: v, d
+  for (int t = threadIdx.x; t < WT * CN; t += WTHREADS) {
+    const int iyl = t % WT, cxl = t / WT;
+    const int par = (iy0 + iyl) & 1;
+    const int kb = iyl >> 1;   // floor(i/2) - cy0 (iy0 even)
+    const float* a0 = cs + (0 * CN + cxl) * CN + kb;
+    const float* h0 = cs + (1 * CN + cxl) * CN + kb;
+    const float* v0 = cs + (2 * CN + cxl) * CN + kb;
+    const float* d0 = cs + (3 * CN + cxl) * CN + kb;
+    float sl = 0.f, sh = 0.f;
+#pragma unroll
+    for (int m = 0; m < LF / 2; ++m) {
+      const int j = par + LF - 2 - 2 * m;
+      const float fa = rlo[j], fh = rhi[j];
+      sl += fa * a0[m] + fh * h0[m];
+      sh += fa * v0[m] + fh * d0[m];
+    }
+    Ly[cxl * WT + iyl] = sl;
+    Hy[cxl * WT + iyl] = sh;
+  }
+  __syncthreads();
+  // x pass
+  int q = 0;
+  for (int t = threadIdx.x; t < WT * WT; t += WTHREADS, ++q) {
+    const int iyl = t % WT, ixl = t / WT;
+    const int par = (ix0 + ixl) & 1;
+    const int kb = ixl >> 1;
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < LF / 2; ++m) {
+      const int j = par + LF - 2 - 2 * m;
+      s += rlo[j] * Ly[(kb + m) * WT + iyl] + rhi[j] * Hy[(kb + m) * WT + iyl];
+    }
+    if (iy0 + iyl < py && ix0 + ixl < px) acc[q] += s;
+  }
+  __syncthreads();
+}
+
+__device__ void syn_tile(const SynIn& in, int lf, const float* lo, const float* hi, int iy0, int ix0, int py,
+                         int px, float* cs, float* Ly, float* Hy, float* acc) {
+  switch (lf) {
+    case 2: syn_tile_t<2>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    case 4: syn_tile_t<4>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    case 6: syn_tile_t<6>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    case 8: syn_tile_t<8>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    case 10: syn_tile_t<10>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    case 12: syn_tile_t<12>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    case 14: syn_tile_t<14>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+    default: syn_tile_t<16>(in, lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc); break;
+  }
+}
+
+// runtime filter length version (the finest level, whose basis loop would
+// otherwise hold all eight unrolled instances): accumulates the tile into acc
+__device__ void syn_tile_dyn(const SynIn& in, int lf, const float* lo, const float* hi, int iy0, int ix0,
                          int py, int px, float* cs, float* Ly, float* Hy, float* acc) {
   const int cy0 = (iy0 - 1) >= 0 ? (iy0) / 2 : 0;    // ceil((iy0-1)/2) for iy0 >= 0
   const int cx0 = (ix0 - 1) >= 0 ? (ix0) / 2 : 0;
@@ -318,7 +433,7 @@ __global__ __launch_bounds__(WTHREADS) void k_syn_l1(SaraGeom G, const float* __
     in.h = v1l + G.boff[b] + G.doff[b][1];
     in.ny = G.ny[b][1];
     in.nx = G.nx[b][1];
-    syn_tile(in, lf, flo, fhi, iy0, ix0, G.Ny, G.Nx, cs, Ly, Hy, acc);
+    syn_tile_dyn(in, lf, flo, fhi, iy0, ix0, G.Ny, G.Nx, cs, Ly, Hy, acc);
   }
   const float* dir = v1l + G.boff[G.nb];
   float* o = out + (int64_t)l * G.Ny * G.Nx;
