@@ -449,6 +449,54 @@ __global__ __launch_bounds__(WTHREADS) void k_syn_l1(SaraGeom G, const float* __
   }
 }
 
+// The standard SARA dictionary (db1..db8 in order: filter length 2(b+1)),
+// finest level: the basis loop is unrolled at compile time so every basis
+// runs its fixed-length tile code.
+template <int B>
+__device__ __forceinline__ void syn_l1_std_bases(const SaraGeom& G, const float* v1l, const float* sc, int iy0,
+                                                 int ix0, float* flo, float* fhi, float* cs, float* Ly, float* Hy,
+                                                 float* acc) {
+  constexpr int LF = 2 * (B + 1);
+  if (threadIdx.x < LF) {
+    flo[threadIdx.x] = G.lo[B][threadIdx.x];
+    fhi[threadIdx.x] = G.hi[B][threadIdx.x];
+  }
+  SynIn in;
+  in.a = G.nlevel == 1 ? v1l + G.boff[B] + G.aoff[B] : sc + G.scr[B][1];
+  in.h = v1l + G.boff[B] + G.doff[B][1];
+  in.ny = G.ny[B][1];
+  in.nx = G.nx[B][1];
+  syn_tile_t<LF>(in, flo, fhi, iy0, ix0, G.Ny, G.Nx, cs, Ly, Hy, acc);
+  if constexpr (B + 1 < WMAX_BASES) syn_l1_std_bases<B + 1>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, acc);
+}
+
+__global__ __launch_bounds__(WTHREADS) void k_syn_l1_std(SaraGeom G, const float* __restrict__ v1,
+                                                         const float* __restrict__ scr, float scale,
+                                                         float* __restrict__ out) {
+  __shared__ float cs[4 * CMAX * CMAX];
+  __shared__ float Ly[CMAX * WT];
+  __shared__ float Hy[CMAX * WT];
+  __shared__ float flo[WMAX_LF], fhi[WMAX_LF];
+  const int l = blockIdx.z;
+  const int iy0 = blockIdx.x * WT, ix0 = blockIdx.y * WT;
+  const float* v1l = v1 + (int64_t)l * G.s;
+  const float* sc = scr + (int64_t)l * G.scr_per_chan;
+  float acc[WT * WT / WTHREADS] = {0.f, 0.f, 0.f, 0.f};
+  syn_l1_std_bases<0>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, acc);
+  const float* dir = v1l + G.boff[G.nb];
+  float* o = out + (int64_t)l * G.Ny * G.Nx;
+  int q = 0;
+  for (int t = threadIdx.x; t < WT * WT; t += WTHREADS, ++q) {
+    const int iy = iy0 + t % WT, ix = ix0 + t / WT;
+    if (iy < G.Ny && ix < G.Nx) {
+      const int64_t p = iy + (int64_t)G.Ny * ix;
+      float s = acc[q] * G.norm;
+      if (G.dirac) s += dir[p] * G.norm;
+      o[p] = scale * s;
+    }
+  }
+}
+
 // ---- l21 dual step, Dirac analysis fused ------------------------------------
 // r_l = v1_l (+ xhat_l(p)/sqrt(P) on the Dirac block); l2 = ||r||_2 over l;
 // v1 <- r - max(l2 - beta w, 0)/(l2 + eps) r.   mode 1: only sum l2 (prior).
@@ -561,7 +609,12 @@ void sara_synthesis(fhs_ctx* ctx, const fhs_sara* S, const float* v1, int L, flo
     k_syn_lk<<<dim3(cdiv(py, WT), cdiv(px, WT), G.nb * L), WTHREADS, 0, ctx->stream>>>(G, k, v1, scr);
     check_launch();
   }
-  k_syn_l1<<<dim3(cdiv(G.Ny, WT), cdiv(G.Nx, WT), L), WTHREADS, 0, ctx->stream>>>(G, v1, scr, scale, out);
+  bool std_dict = G.nb == WMAX_BASES;
+  for (int b = 0; b < G.nb && std_dict; ++b) std_dict = G.lf[b] == 2 * (b + 1);
+  if (std_dict)
+    k_syn_l1_std<<<dim3(cdiv(G.Ny, WT), cdiv(G.Nx, WT), L), WTHREADS, 0, ctx->stream>>>(G, v1, scr, scale, out);
+  else
+    k_syn_l1<<<dim3(cdiv(G.Ny, WT), cdiv(G.Nx, WT), L), WTHREADS, 0, ctx->stream>>>(G, v1, scr, scale, out);
   check_launch();
 }
 
