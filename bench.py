@@ -51,7 +51,15 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
-    return ap.parse_args()
+    ap.add_argument('--config', choices=('hs512', 'fhs1024'), default='hs512',
+                    help='hs512: HyperSARA 512x512x16 (the headline workload); fhs1024: one 16-channel spectral '
+                         'sub-cube of Faceted HyperSARA 1024x1024x32 (C = 2), 4x4 nuclear-norm facets with 25%% '
+                         'overlap, 300k-500k visibilities per channel')
+    a = ap.parse_args()
+    if a.config == 'fhs1024':
+        a.size, a.channels, a.vis = 1024, 16, 500000
+        a.no_cpu = True     # one FP64 oracle iteration of this size takes minutes
+    return a
 
 
 def algorithmic_bytes(meas, sara_s):
@@ -112,7 +120,15 @@ def main():
         return
     ctx = fhs.default_context(0)
     t_setup = time.perf_counter()
-    prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
+    facets = None
+    if args.config == 'fhs1024':
+        from fhs.lowrank import facet_geometry
+        prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed,
+                                  flag_range=(0.0, 0.4))
+        facets = facet_geometry(args.size, args.size, 4, 4, (0.25, 0.25))
+    else:
+        prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
+    kind = 'fhs' if facets is not None else 'hs'
     meas = solvers.Measurement(ctx, prob, precond=True)
     son = meas.op_norm(False)
     nu2 = meas.op_norm(True)
@@ -126,7 +142,7 @@ def main():
         holder['eng'] = eng
         return t < args.warmup
 
-    solvers.hyper_sara(meas, param, callback=grab, max_total_iter=args.warmup + 1)
+    solvers._solve(kind, meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1)
     eng = holder['eng']
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -156,7 +172,8 @@ def main():
     # (copy-calibrated) for this workload, when that summary exists
     traffic = None
     tpath = os.path.join(ROOT, 'profiles', 'hs512x16_pmc_traffic.json')
-    if (args.size, args.channels, args.vis, args.seed) == (512, 16, 200000, 0) and os.path.exists(tpath):
+    if args.config == 'hs512' and (args.size, args.channels, args.vis, args.seed) == (512, 16, 200000, 0) \
+            and os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get('traffic_bytes_per_launch', {}).get(dominant)
     kernels = {k: {'ms_per_launch': rep[k][0] / rep[k][1], 'launches_per_iter': rep[k][1] / n_timed,
@@ -168,7 +185,7 @@ def main():
     if not args.no_solve:
         from fhs import warmstart
         prm = dict(param, track_snr=True)
-        _, plain = solvers._solve('hs', meas, prm, X0=prob.X0, time_limit=args.solve_seconds)
+        _, plain = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds)
         ref = plain['snr'][-1]
         tols = (0.1, 0.5, 1.0, 2.0, 3.0)
 
@@ -197,7 +214,7 @@ def main():
             x0_snr = 20 * np.log10(torch.linalg.vector_norm(X0d).item() /
                                    torch.linalg.vector_norm(X0d - x0.double()).item())
             target = ref - 0.1
-            _, warm = solvers._solve('hs', meas, prm, X0=prob.X0, x_init=x0, weights_from_init=False,
+            _, warm = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, x_init=x0, weights_from_init=False,
                                      time_limit=args.solve_seconds)
             solve['warm_start'] = {'network': os.path.relpath(wpath, ROOT), 'predict_s': round(t_pred, 4),
                                    'x_init_snr_db': round(x0_snr, 3),
@@ -253,8 +270,12 @@ def main():
         'vs_baseline': None,
         'dtype': 'f32 (f64 norms, f64 Gram/eig)',
         'data': 'synthetic',
-        'config': {'workload': f'HyperSARA {args.size}x{args.size}x{args.channels}, '
-                               f'{args.vis} base vis/channel, 10-30% flags, 4 blocks/channel',
+        'config': {'workload': (f'HyperSARA {args.size}x{args.size}x{args.channels}, '
+                                f'{args.vis} base vis/channel, 10-30% flags, 4 blocks/channel')
+                   if facets is None else
+                   (f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 2 spectral sub-cubes '
+                    f'of {args.size}x{args.size}x{2 * args.channels}), 4x4 nuclear-norm facets, 25% overlap, '
+                    f'{int(args.vis * 0.6)}-{args.vis} vis/channel, 4 blocks/channel'),
                    'Mtot': int(meas.D.Mtot), 'nnz_G_total': int(sum(g.nnz for g in meas.G)),
                    'setup_s': round(setup_s, 2)},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1),

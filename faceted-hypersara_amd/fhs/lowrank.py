@@ -8,6 +8,46 @@ from ._lib import lib
 from .core import _ptr, _require
 
 
+def facet_geometry(Ny, Nx, Qy, Qx, overlap_fraction=(0.0, 0.0)):
+    """Overlapping nuclear-norm facets of an Ny x Nx image (facet q = qx*Qy + qy).
+
+    Plain facet boundaries are round(i N / Q) (MATLAB rounding) as the
+    reference's ``split_range`` gives them (``src/fhs/facetHyperSARA.m:297-321``);
+    every facet after the first along an axis starts ``floor(f N / Q)``
+    pixels earlier (``get_overlap_size``, ``imaging/imaging.m:421``), and the
+    apodisation window is a separable ramp that rises over the leading
+    overlap strip and falls over the trailing one, so overlapping windows sum
+    to one (the 'triangular' ``generate_weights`` of
+    ``src/fhs/fhs_setup_priors.m:189``).  Returns the dict ``LowRankPrior``
+    takes: Qy, Qx, oy, ox, py, px, windows, overlap."""
+    def axis(N, Q, f):
+        d = int(np.floor(f * N / Q)) if Q > 1 else 0
+        edge = [int(np.floor(i * N / Q + 0.5)) for i in range(Q + 1)]
+        spans = [(max(edge[q] - (d if q > 0 else 0), 0), edge[q + 1]) for q in range(Q)]
+        wins = []
+        for q, (a, b) in enumerate(spans):
+            w = np.ones(b - a)
+            lead = d if q > 0 else 0
+            trail = d if q < Q - 1 else 0
+            w[:lead] = np.arange(1, lead + 1) / (lead + 1)
+            if trail:
+                w[b - a - trail:] = 1.0 - np.arange(1, trail + 1) / (trail + 1)
+            wins.append(w)
+        return d, spans, wins
+
+    dy, sy, wy = axis(Ny, Qy, overlap_fraction[0])
+    dx, sx, wx = axis(Nx, Qx, overlap_fraction[1])
+    g = {'Qy': Qy, 'Qx': Qx, 'oy': [], 'ox': [], 'py': [], 'px': [], 'windows': [], 'overlap': (dy, dx)}
+    for qx in range(Qx):
+        for qy in range(Qy):
+            g['oy'].append(sy[qy][0])
+            g['ox'].append(sx[qx][0])
+            g['py'].append(sy[qy][1] - sy[qy][0])
+            g['px'].append(sx[qx][1] - sx[qx][0])
+            g['windows'].append(np.outer(wy[qy], wx[qx]))
+    return g
+
+
 class LowRankPrior:
     """Facet layout of the nuclear-norm prior (``fhs_lowrank``).
 

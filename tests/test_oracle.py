@@ -328,3 +328,16 @@ def test_faceted_lowrank_single_facet_is_global():
     v0s, g0 = ofc.faceted_lowrank_step([np.zeros((120, 3))], x, geom, np.ones((1, 3)), 0.5, 1.0)
     ref = opx.nuclear_prox_dual(x.reshape(120, 3, order='F'), np.full(3, 0.5))
     assert np.allclose(v0s[0], ref) and np.allclose(g0, ref.reshape(12, 10, 3, order='F'))
+
+
+@pytest.mark.parametrize('N,Q,f', [(64, 2, 0.25), (1024, 4, 0.25), (1000, 3, 0.5), (512, 1, 0.25), (77, 5, 0.1)])
+def test_product_facet_geometry_matches_oracle(N, Q, f):
+    """The product's facet builder (used by the faceted solver and the bench)
+    equals the oracle's restatement, windows included."""
+    from fhs.lowrank import facet_geometry as product
+    from oracle.facets import facet_geometry as ref
+    a, b = product(N, N // 2 + 1, Q, max(Q - 1, 1), (f, f)), ref(N, N // 2 + 1, Q, max(Q - 1, 1), (f, f))
+    for k in ('Qy', 'Qx', 'oy', 'ox', 'py', 'px', 'overlap'):
+        assert list(np.atleast_1d(a[k])) == list(np.atleast_1d(b[k])), k
+    for wa, wb in zip(a['windows'], b['windows']):
+        assert np.array_equal(wa, wb)
