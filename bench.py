@@ -205,6 +205,7 @@ def main():
         wpath = warmstart.default_weights_path()
         if os.path.exists(wpath):
             ws = warmstart.WarmStart.load(ctx, wpath)
+            ws.predict(warmstart.dirty_cube(meas))      # one untimed call: workspaces and code objects
             torch.cuda.synchronize()
             tw = time.perf_counter()
             x0 = ws.predict(warmstart.dirty_cube(meas))

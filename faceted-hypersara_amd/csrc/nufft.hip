@@ -106,7 +106,7 @@ __device__ __forceinline__ void fft_lds(float2* s, const float2* __restrict__ tw
 template <int N>
 constexpr int nt_contig() { return N / 4 < 64 ? 64 : (N / 4 > 1024 ? 1024 : N / 4); }
 template <int N>
-constexpr int b_strided() { return N >= 4096 ? 4 : (N >= 2048 ? 8 : 16); }
+constexpr int b_strided() { return N >= 4096 ? 2 : (N >= 2048 ? 4 : (N >= 1024 ? 8 : 16)); }
 template <int N>
 constexpr int nt_strided() {
   return b_strided<N>() * N / 4 > 1024 ? 1024 : (b_strided<N>() * N / 4 < 64 ? 64 : b_strided<N>() * N / 4);
