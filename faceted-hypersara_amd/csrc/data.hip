@@ -140,6 +140,9 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
   pmax = 0.f;
   for (int w = 0; w < FB_THREADS / 64; ++w) pmax = fmaxf(pmax, shm[w]);
   const float pm = pmax > 0.f ? pmax : 1.f;
+  const float inv_pm = 1.f / pm;
+  // a = pU / max pU as one multiply; exactly 1 where pU is maximal
+  auto step = [&](float p) { return p == pm ? 1.f : p * inv_pm; };
   // gradient step z - mu pU (z - w), mu = 1/max pU, as (1-a) z + a w with
   // a = pU / max pU (a == 1 exactly where pU is maximal, so a constant pU
   // reproduces w bit for bit and the iteration stops at its fixed point).
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
     const float2 vv = v2[i], rr = r2[i], zz = z[i], yy = y[i];
     const float2 w = make_float2(vv.x / p + rr.x, vv.y / p + rr.y);
     v2[i] = w;
-    const float aa = p / pm;
+    const float aa = step(p);
     const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
     const double dx = (double)g.x - yy.x, dy = (double)g.y - yy.y;
     s1[0] += dx * dx + dy * dy;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
     for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
       const float p = pU ? pU[i] : 1.f;
       const float2 w = v2[i], zz = z[i], yy = y[i];
-      const float aa = p / pm;
+      const float aa = step(p);
       const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
       const float2 zn = make_float2(yy.x + (g.x - yy.x) * scale, yy.y + (g.y - yy.y) * scale);
       z[i] = zn;

@@ -511,9 +511,19 @@ __global__ __launch_bounds__(256) void k_l21(SaraGeom G, float* __restrict__ v1,
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s;
        c += (int64_t)gridDim.x * blockDim.x) {
     const bool dir = G.dirac && c >= dstart;
+    constexpr int LREG = 16;   // channels kept in registers between the norm and the update
+    float rv[LREG];
     double ss = 0.0;
-#pragma unroll 8
-    for (int l = 0; l < L; ++l) {
+#pragma unroll
+    for (int l = 0; l < LREG; ++l) {
+      if (l < L) {
+        float r = v1[(int64_t)l * s + c];
+        if (dir && xhat) r += xhat[(int64_t)l * G.Ny * G.Nx + (c - dstart)] * G.norm;
+        rv[l] = r;
+        ss += (double)r * r;
+      }
+    }
+    for (int l = LREG; l < L; ++l) {
       float r = v1[(int64_t)l * s + c];
       if (dir && xhat) r += xhat[(int64_t)l * G.Ny * G.Nx + (c - dstart)] * G.norm;
       ss += (double)r * r;
@@ -529,8 +539,10 @@ __global__ __launch_bounds__(256) void k_l21(SaraGeom G, float* __restrict__ v1,
     }
     const double t = beta * (w ? (double)w[c] : 1.0);
     const float f = (float)(fmax(l2 - t, 0.0) / (l2 + 2.220446049250313e-16));
-#pragma unroll 8
-    for (int l = 0; l < L; ++l) {
+#pragma unroll
+    for (int l = 0; l < LREG; ++l)
+      if (l < L) v1[(int64_t)l * s + c] = rv[l] - f * rv[l];
+    for (int l = LREG; l < L; ++l) {
       float r = v1[(int64_t)l * s + c];
       if (dir && xhat) r += xhat[(int64_t)l * G.Ny * G.Nx + (c - dstart)] * G.norm;
       v1[(int64_t)l * s + c] = r - f * r;
