@@ -324,11 +324,22 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
   float2 acc[TILE_PTS];
 #pragma unroll
   for (int i = 0; i < TILE_PTS; ++i) acc[i] = make_float2(0.f, 0.f);
+  // the next chunk's row index and cell are loaded one chunk ahead, so the
+  // row-indexed loads of a chunk do not wait behind the index load
+  const int i = threadIdx.x;
+  int r_next = 0, cell_next = 0;
+  if (p0 + i < p1) {
+    r_next = g.tile_rows[p0 + i];
+    cell_next = g.tile_cell[p0 + i];
+  }
   for (int c0 = p0; c0 < p1; c0 += TILE_CH) {
     const int n = min(TILE_CH, p1 - c0);
-    const int i = threadIdx.x;
+    const int r = r_next, rcell = cell_next;
+    if (c0 + TILE_CH + i < p1) {
+      r_next = g.tile_rows[c0 + TILE_CH + i];
+      cell_next = g.tile_cell[c0 + TILE_CH + i];
+    }
     if (i < n) {
-      const int r = g.tile_rows[c0 + i];
       const float2 v = w(r);
       float2 cy[SEP_KMAX], cx[SEP_KMAX];
       sep_load(g, r, cy, cx);
@@ -341,7 +352,7 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
         }
         if (j < g.Ky) q[g.Kx + j] = make_float2(cy[j].x, -cy[j].y);
       }
-      s.cell[i] = g.tile_cell[c0 + i];
+      s.cell[i] = rcell;
     }
     for (int c = threadIdx.x; c <= ncells; c += ADJ_THREADS) s.start[c] = n;
     __syncthreads();
@@ -430,8 +441,11 @@ __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const fl
     if (e < nw) win[e] = v[i];
   }
   __syncthreads();
-  for (int k = it.y + threadIdx.x; k < it.z; k += ADJ_THREADS) {
-    const int r = g.prim_rows[k];
+  int k = it.y + threadIdx.x;
+  int rn = k < it.z ? g.prim_rows[k] : 0;
+  for (; k < it.z; k += ADJ_THREADS) {
+    const int r = rn;   // the next row index is loaded one step ahead
+    if (k + ADJ_THREADS < it.z) rn = g.prim_rows[k + ADJ_THREADS];
     const int a = g.anchor[r];
     const int ly = a % g.Kty - ty0, lx = a / g.Kty - tx0;
     float2 cy[SEP_KMAX], cx[SEP_KMAX];

@@ -9,7 +9,7 @@
 namespace fhs {
 namespace {
 
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 2048;
 constexpr int RED_THREADS = 256;
 
 __global__ __launch_bounds__(RED_THREADS) void k_sqdiff_partial(const float2* __restrict__ x,

@@ -67,9 +67,34 @@ struct AnOut {
   bool last;
 };
 
+constexpr int AN_PER = WT * WT / WTHREADS;   // outputs per thread
+
+// old values of the coefficients a tile accumulates into (+=), loaded at
+// kernel start so that their latency overlaps the window load
+struct AnPre {
+  float a[AN_PER], h[AN_PER], v[AN_PER], d[AN_PER];
+};
+
+__device__ __forceinline__ void an_prefetch(int ky0, int kx0, const AnOut& out, AnPre& pre) {
+#pragma unroll
+  for (int q = 0; q < AN_PER; ++q) {
+    const int t = threadIdx.x + q * WTHREADS;
+    const int gy = ky0 + t % WT, gx = kx0 + t / WT;
+    pre.a[q] = pre.h[q] = pre.v[q] = pre.d[q] = 0.f;
+    if (gy < out.ny && gx < out.nx) {
+      const int64_t p = gy + (int64_t)out.ny * gx;
+      const int64_t sz = (int64_t)out.ny * out.nx;
+      if (out.last) pre.a[q] = out.v1[out.a_off + p];
+      pre.h[q] = out.v1[out.h_off + p];
+      pre.v[q] = out.v1[out.h_off + sz + p];
+      pre.d[q] = out.v1[out.h_off + 2 * sz + p];
+    }
+  }
+}
+
 template <int LF>
 __device__ void an_tile_t(const float* win, int WH, const float* lo, const float* hi, int ky0, int kx0, float* Lx,
-                          float* Hx, const AnOut& out) {
+                          float* Hx, const AnOut& out, const AnPre& pre) {
   constexpr int H = 2 * WT + LF - 2;   // rows of the window
   float rlo[LF], rhi[LF];
 #pragma unroll
@@ -93,7 +118,9 @@ __device__ void an_tile_t(const float* win, int WH, const float* lo, const float
   }
   __syncthreads();
   // y pass
-  for (int t = threadIdx.x; t < WT * WT; t += WTHREADS) {
+#pragma unroll
+  for (int q = 0; q < AN_PER; ++q) {
+    const int t = threadIdx.x + q * WTHREADS;
     const int ky = t % WT, kx = t / WT;
     const int gy = ky0 + ky, gx = kx0 + kx;
     if (gy < out.ny && gx < out.nx) {
@@ -110,12 +137,12 @@ __device__ void an_tile_t(const float* win, int WH, const float* lo, const float
       const int64_t p = gy + (int64_t)out.ny * gx;
       const int64_t sz = (int64_t)out.ny * out.nx;
       if (out.last)
-        out.v1[out.a_off + p] += a;
+        out.v1[out.a_off + p] = pre.a[q] + a;
       else
         out.approx[out.a_off + p] = a;
-      out.v1[out.h_off + p] += h;
-      out.v1[out.h_off + sz + p] += v;
-      out.v1[out.h_off + 2 * sz + p] += d;
+      out.v1[out.h_off + p] = pre.h[q] + h;
+      out.v1[out.h_off + sz + p] = pre.v[q] + v;
+      out.v1[out.h_off + 2 * sz + p] = pre.d[q] + d;
     }
   }
   __syncthreads();
@@ -123,16 +150,16 @@ __device__ void an_tile_t(const float* win, int WH, const float* lo, const float
 
 // filter lengths of db1..db8 are 2, 4, ..., 16: one unrolled instance each
 __device__ void an_tile(const float* win, int WH, int lf, const float* lo, const float* hi, int ky0, int kx0,
-                        float* Lx, float* Hx, const AnOut& out) {
+                        float* Lx, float* Hx, const AnOut& out, const AnPre& pre) {
   switch (lf) {
-    case 2: an_tile_t<2>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    case 4: an_tile_t<4>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    case 6: an_tile_t<6>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    case 8: an_tile_t<8>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    case 10: an_tile_t<10>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    case 12: an_tile_t<12>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    case 14: an_tile_t<14>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
-    default: an_tile_t<16>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out); break;
+    case 2: an_tile_t<2>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    case 4: an_tile_t<4>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    case 6: an_tile_t<6>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    case 8: an_tile_t<8>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    case 10: an_tile_t<10>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    case 12: an_tile_t<12>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    case 14: an_tile_t<14>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
+    default: an_tile_t<16>(win, WH, lo, hi, ky0, kx0, Lx, Hx, out, pre); break;
   }
 }
 
@@ -180,12 +207,6 @@ __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const flo
   }
   const int WH = 2 * WT + lf - 2;
   float* sc = scr + (int64_t)l * G.scr_per_chan;
-  if (k == 1)   // level 1 reads the image, scaled by 1/sqrt(P)
-    load_window(win, x + (int64_t)l * G.Ny * G.Nx, G.Ny, G.Nx, 2 * ky0 + 2 - lf, 2 * kx0 + 2 - lf, WH, WH,
-                G.norm);
-  else
-    load_window(win, sc + G.scr[b][k - 1], G.ny[b][k - 1], G.nx[b][k - 1], 2 * ky0 + 2 - lf,
-                2 * kx0 + 2 - lf, WH, WH, 1.f);
   AnOut out;
   out.v1 = v1 + (int64_t)l * G.s;
   out.last = k == G.nlevel;
@@ -194,7 +215,15 @@ __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const flo
   out.h_off = G.boff[b] + G.doff[b][k];
   out.ny = G.ny[b][k];
   out.nx = G.nx[b][k];
-  an_tile(win, WH, lf, flo, fhi, ky0, kx0, Lx, Hx, out);
+  AnPre pre;
+  an_prefetch(ky0, kx0, out, pre);
+  if (k == 1)   // level 1 reads the image, scaled by 1/sqrt(P)
+    load_window(win, x + (int64_t)l * G.Ny * G.Nx, G.Ny, G.Nx, 2 * ky0 + 2 - lf, 2 * kx0 + 2 - lf, WH, WH,
+                G.norm);
+  else
+    load_window(win, sc + G.scr[b][k - 1], G.ny[b][k - 1], G.nx[b][k - 1], 2 * ky0 + 2 - lf,
+                2 * kx0 + 2 - lf, WH, WH, 1.f);
+  an_tile(win, WH, lf, flo, fhi, ky0, kx0, Lx, Hx, out, pre);
 }
 
 // ---- synthesis -------------------------------------------------------------
@@ -209,22 +238,24 @@ struct SynIn {
   int ny, nx;
 };
 
-// accumulates the tile (WT*WT/WTHREADS values per thread) into acc.
+// A synthesis tile in three steps, so that a caller can overlap the global
+// loads of the next tile with the arithmetic of the current one:
+//   syn_load_t   issues the loads of the 4 subband tiles (CN x CN) into registers,
+//   syn_store_t  writes them to LDS (cs[s][cx][cy]; a barrier must follow),
+//   syn_compute_t accumulates the tile (WT*WT/WTHREADS values per thread) into acc.
 // Output i takes the LF/2 coefficients k = floor(i/2) + m (m < LF/2) with
 // filter tap j = (i & 1) + LF - 2 - 2m; the staged tile is zero beyond the
 // subband, so no clamping is needed.
+constexpr int CS_MAX = (4 * CMAX * CMAX + WTHREADS - 1) / WTHREADS;   // registers per thread
+
 template <int LF>
-__device__ void syn_tile_t(const SynIn& in, const float* lo, const float* hi, int iy0, int ix0, int py, int px,
-                           float* cs, float* Ly, float* Hy, float* acc) {
+__device__ __forceinline__ void syn_load_t(const SynIn& in, int iy0, int ix0, float* cv) {
   constexpr int CN = WT / 2 + LF / 2 + 1;
+  constexpr int NCS = 4 * CN * CN;
   const int cy0 = iy0 / 2, cx0 = ix0 / 2;
   const int64_t sz = (int64_t)in.ny * in.nx;
-  // load 4 subband tiles CN x CN: cs[s][cx][cy] (all loads issued first)
-  constexpr int NCS = 4 * CN * CN;
-  constexpr int CS_PER_THREAD = (NCS + WTHREADS - 1) / WTHREADS;
-  float cv[CS_PER_THREAD];
 #pragma unroll
-  for (int i = 0; i < CS_PER_THREAD; ++i) {
+  for (int i = 0; i < CS_MAX; ++i) {
     const int t = threadIdx.x + i * WTHREADS;
     cv[i] = 0.f;
     if (t < NCS) {
@@ -236,13 +267,24 @@ __device__ void syn_tile_t(const SynIn& in, const float* lo, const float* hi, in
       }
     }
   }
+}
+
+template <int LF>
+__device__ __forceinline__ void syn_store_t(const float* cv, float* cs) {
+  constexpr int CN = WT / 2 + LF / 2 + 1;
+  constexpr int NCS = 4 * CN * CN;
 #pragma unroll
-  for (int i = 0; i < CS_PER_THREAD; ++i) {
+  for (int i = 0; i < CS_MAX; ++i) {
     const int t = threadIdx.x + i * WTHREADS;
     if (t < NCS) cs[t] = cv[i];
   }
-  __syncthreads();
-  // the caller's filter writes are visible after the barrier above
+}
+
+// expects cs staged and the caller's filter writes visible (after a barrier)
+template <int LF>
+__device__ __forceinline__ void syn_compute_t(const float* lo, const float* hi, int iy0, int ix0, int py, int px,
+                                              const float* cs, float* Ly, float* Hy, float* acc) {
+  constexpr int CN = WT / 2 + LF / 2 + 1;
   float rlo[LF], rhi[LF];
 #pragma unroll
   for (int j = 0; j < LF; ++j) {
@@ -285,6 +327,16 @@ __device__ void syn_tile_t(const SynIn& in, const float* lo, const float* hi, in
     if (iy0 + iyl < py && ix0 + ixl < px) acc[q] += s;
   }
   __syncthreads();
+}
+
+template <int LF>
+__device__ void syn_tile_t(const SynIn& in, const float* lo, const float* hi, int iy0, int ix0, int py, int px,
+                           float* cs, float* Ly, float* Hy, float* acc) {
+  float cv[CS_MAX];
+  syn_load_t<LF>(in, iy0, ix0, cv);
+  syn_store_t<LF>(cv, cs);
+  __syncthreads();
+  syn_compute_t<LF>(lo, hi, iy0, ix0, py, px, cs, Ly, Hy, acc);
 }
 
 __device__ void syn_tile(const SynIn& in, int lf, const float* lo, const float* hi, int iy0, int ix0, int py,
@@ -451,23 +503,32 @@ __global__ __launch_bounds__(WTHREADS) void k_syn_l1(SaraGeom G, const float* __
 
 // The standard SARA dictionary (db1..db8 in order: filter length 2(b+1)),
 // finest level: the basis loop is unrolled at compile time so every basis
-// runs its fixed-length tile code.
+// runs its fixed-length tile code, and the subband loads of basis B+1 are
+// issued before the arithmetic of basis B (software pipelining).
+__device__ __forceinline__ SynIn syn_l1_in(const SaraGeom& G, const float* v1l, const float* sc, int b) {
+  SynIn in;
+  in.a = G.nlevel == 1 ? v1l + G.boff[b] + G.aoff[b] : sc + G.scr[b][1];
+  in.h = v1l + G.boff[b] + G.doff[b][1];
+  in.ny = G.ny[b][1];
+  in.nx = G.nx[b][1];
+  return in;
+}
+
 template <int B>
 __device__ __forceinline__ void syn_l1_std_bases(const SaraGeom& G, const float* v1l, const float* sc, int iy0,
                                                  int ix0, float* flo, float* fhi, float* cs, float* Ly, float* Hy,
-                                                 float* acc) {
+                                                 float* cv, float* acc) {
   constexpr int LF = 2 * (B + 1);
   if (threadIdx.x < LF) {
     flo[threadIdx.x] = G.lo[B][threadIdx.x];
     fhi[threadIdx.x] = G.hi[B][threadIdx.x];
   }
-  SynIn in;
-  in.a = G.nlevel == 1 ? v1l + G.boff[B] + G.aoff[B] : sc + G.scr[B][1];
-  in.h = v1l + G.boff[B] + G.doff[B][1];
-  in.ny = G.ny[B][1];
-  in.nx = G.nx[B][1];
-  syn_tile_t<LF>(in, flo, fhi, iy0, ix0, G.Ny, G.Nx, cs, Ly, Hy, acc);
-  if constexpr (B + 1 < WMAX_BASES) syn_l1_std_bases<B + 1>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, acc);
+  syn_store_t<LF>(cv, cs);   // basis B's subbands, loaded one step ahead
+  __syncthreads();
+  if constexpr (B + 1 < WMAX_BASES) syn_load_t<2 * (B + 2)>(syn_l1_in(G, v1l, sc, B + 1), iy0, ix0, cv);
+  syn_compute_t<LF>(flo, fhi, iy0, ix0, G.Ny, G.Nx, cs, Ly, Hy, acc);
+  if constexpr (B + 1 < WMAX_BASES)
+    syn_l1_std_bases<B + 1>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, cv, acc);
 }
 
 __global__ __launch_bounds__(WTHREADS) void k_syn_l1_std(SaraGeom G, const float* __restrict__ v1,
@@ -482,7 +543,9 @@ __global__ __launch_bounds__(WTHREADS) void k_syn_l1_std(SaraGeom G, const float
   const float* v1l = v1 + (int64_t)l * G.s;
   const float* sc = scr + (int64_t)l * G.scr_per_chan;
   float acc[WT * WT / WTHREADS] = {0.f, 0.f, 0.f, 0.f};
-  syn_l1_std_bases<0>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, acc);
+  float cv[CS_MAX];
+  syn_load_t<2>(syn_l1_in(G, v1l, sc, 0), iy0, ix0, cv);
+  syn_l1_std_bases<0>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, cv, acc);
   const float* dir = v1l + G.boff[G.nb];
   float* o = out + (int64_t)l * G.Ny * G.Nx;
   int q = 0;
