@@ -359,11 +359,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_forward_tiled(const GDev* 
   if (!g.tiled || u >= g.nfitems) return;
   float2* __restrict__ go = gx_old + row_off[l];
   float2* __restrict__ rr = r2 + row_off[l];
-  g_forward_tile(g, u, Fx + (size_t)l * No, win, [=](int r, float2 gx) {
-    const float2 old = go[r];
-    rr[r] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
-    go[r] = gx;
-  });
+  g_forward_tile(
+      g, u, Fx + (size_t)l * No, win, [=](int r) { return go[r]; },
+      [=](int r, float2 gx, float2 old) {
+        rr[r] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
+        go[r] = gx;
+      });
 }
 
 __global__ __launch_bounds__(ADJ_THREADS) void k_data_apply_tiled(const GDev* __restrict__ gd,
@@ -378,8 +379,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_apply_tiled(const GDev* __
   if (!g.tiled || u >= g.nfitems) return;
   float2* __restrict__ yo = y + row_off[l];
   const float* __restrict__ lm = lam ? lam + row_off[l] : nullptr;
-  g_forward_tile(g, u, Fx + (size_t)l * No, win,
-                 [=](int r, float2 v) { yo[r] = lm ? cscale(v, lm[r]) : v; });
+  g_forward_tile(
+      g, u, Fx + (size_t)l * No, win, [=](int r) { return lm ? lm[r] : 1.f; },
+      [=](int r, float2 v, float s) { yo[r] = lm ? cscale(v, s) : v; });
 }
 
 // G Fx -> r2 = 2 G Fx - gx_old, gx_old <- G Fx (non-DR form) for all channels

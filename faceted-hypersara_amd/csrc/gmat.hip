@@ -150,7 +150,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_forward_tiled(GDev g, cons
                                                                   const float* __restrict__ lam,
                                                                   float2* __restrict__ y) {
   __shared__ float2 win[FWD_WIN_MAX];
-  g_forward_tile(g, blockIdx.x, Fx, win, [&](int r, float2 v) { y[r] = lam ? cscale(v, lam[r]) : v; });
+  g_forward_tile(
+      g, blockIdx.x, Fx, win, [&](int r) { return lam ? lam[r] : 1.f; },
+      [&](int r, float2 v, float s) { y[r] = lam ? cscale(v, s) : v; });
 }
 
 __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,

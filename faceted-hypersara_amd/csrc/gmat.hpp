@@ -415,9 +415,9 @@ __device__ __forceinline__ void g_adjoint_tile_reduce(const GDev& g, int r, floa
 // before the stores), then every row of the item reads its Ky x Kx taps from
 // LDS instead of gathering them through the caches.  emit(r, G Fx) stores
 // row r's value.
-template <class E>
+template <class P, class E>
 __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const float2* __restrict__ Fx,
-                                               float2* win, E&& emit) {
+                                               float2* win, P&& pre, E&& emit) {
   const int4 it = g.fitems[item];
   const int t = it.x;
   const int ty0 = (t % g.tiles_y) * TILE_Y, tx0 = (t / g.tiles_y) * TILE_X;
@@ -446,6 +446,7 @@ __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const fl
   for (; k < it.z; k += ADJ_THREADS) {
     const int r = rn;   // the next row index is loaded one step ahead
     if (k + ADJ_THREADS < it.z) rn = g.prim_rows[k + ADJ_THREADS];
+    const auto pf = pre(r);   // the caller's per-row loads, issued before the arithmetic
     const int a = g.anchor[r];
     const int ly = a % g.Kty - ty0, lx = a / g.Kty - tx0;
     float2 cy[SEP_KMAX], cx[SEP_KMAX];
@@ -469,7 +470,7 @@ __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const fl
         acc.y += c.x * s.y + c.y * s.x;
       }
     }
-    emit(r, acc);
+    emit(r, acc, pf);
   }
 }
 
