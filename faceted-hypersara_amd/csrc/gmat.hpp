@@ -324,49 +324,35 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
   float2 acc[TILE_PTS];
 #pragma unroll
   for (int i = 0; i < TILE_PTS; ++i) acc[i] = make_float2(0.f, 0.f);
-  // Software pipeline over chunks: the rows of chunk k+1 (their value w(r)
-  // and separable line) are loaded into registers while chunk k is scanned
-  // and gathered, and row indices run two chunks ahead.
+  // the next chunk's row index and cell are loaded one chunk ahead, so the
+  // row-indexed loads of a chunk do not wait behind the index load
   const int i = threadIdx.x;
-  float2 vv = make_float2(0.f, 0.f), cyv[SEP_KMAX], cxv[SEP_KMAX];
-  int cellv = 0;
-  {
-    int r0 = 0;
-    if (p0 + i < p1) {
-      r0 = g.tile_rows[p0 + i];
-      cellv = g.tile_cell[p0 + i];
-      vv = w(r0);
-      sep_load(g, r0, cyv, cxv);
-    }
-  }
   int r_next = 0, cell_next = 0;
-  if (p0 + TILE_CH + i < p1) {
-    r_next = g.tile_rows[p0 + TILE_CH + i];
-    cell_next = g.tile_cell[p0 + TILE_CH + i];
+  if (p0 + i < p1) {
+    r_next = g.tile_rows[p0 + i];
+    cell_next = g.tile_cell[p0 + i];
   }
   for (int c0 = p0; c0 < p1; c0 += TILE_CH) {
     const int n = min(TILE_CH, p1 - c0);
+    const int r = r_next, rcell = cell_next;
+    if (c0 + TILE_CH + i < p1) {
+      r_next = g.tile_rows[c0 + TILE_CH + i];
+      cell_next = g.tile_cell[c0 + TILE_CH + i];
+    }
     if (i < n) {
+      const float2 v = w(r);
+      float2 cy[SEP_KMAX], cx[SEP_KMAX];
+      sep_load(g, r, cy, cx);
       float2* q = s.qc + (size_t)i * s.stride;
 #pragma unroll
       for (int j = 0; j < SEP_KMAX; ++j) {
         if (j < g.Kx) {
-          const float2 c = cxv[j];
-          q[j] = make_float2(c.x * vv.x + c.y * vv.y, c.x * vv.y - c.y * vv.x);   // conj(c) w
+          const float2 c = cx[j];
+          q[j] = make_float2(c.x * v.x + c.y * v.y, c.x * v.y - c.y * v.x);   // conj(c) v
         }
-        if (j < g.Ky) q[g.Kx + j] = make_float2(cyv[j].x, -cyv[j].y);
+        if (j < g.Ky) q[g.Kx + j] = make_float2(cy[j].x, -cy[j].y);
       }
-      s.cell[i] = cellv;
-    }
-    const int c1 = c0 + TILE_CH;
-    if (c1 + i < p1) {   // the next chunk's rows
-      cellv = cell_next;
-      vv = w(r_next);
-      sep_load(g, r_next, cyv, cxv);
-    }
-    if (c1 + TILE_CH + i < p1) {
-      r_next = g.tile_rows[c1 + TILE_CH + i];
-      cell_next = g.tile_cell[c1 + TILE_CH + i];
+      s.cell[i] = rcell;
     }
     for (int c = threadIdx.x; c <= ncells; c += ADJ_THREADS) s.start[c] = n;
     __syncthreads();
