@@ -7,9 +7,11 @@
 namespace fhs {
 
 // xsol <- max(xsol - (ga+gb+gc), 0); xhat = 2 xsol - prev;
-// d_out2[0] = sum (prev - xsol)^2, d_out2[1] = sum xsol^2
+// d_out2[0] = sum (prev - xsol)^2, d_out2[1] = sum xsol^2; with truth
+// (NULL = none) *d_truth = sum (truth - xsol)^2
 void primal_update(fhs_ctx* ctx, float* x, const float* ga, const float* gb, const float* gc,
-                   float* xhat, int64_t n, double* d_out2);
+                   float* xhat, int64_t n, double* d_out2, const float* truth = nullptr,
+                   double* d_truth = nullptr);
 
 // Fx[l] = A(x[l]), l < L
 void nufft_forward(fhs_ctx* ctx, const fhs_nufft* p, const float* x, int L, float2* Fx);

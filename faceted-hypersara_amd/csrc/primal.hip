@@ -47,14 +47,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_proj_apply(const float2* __rest
   }
 }
 
+// With `truth` (a reference cube of the same layout) the kernel also sums
+// (truth - x)^2, so a solver can follow the reconstruction SNR without a
+// separate pass over the cube.
 __global__ __launch_bounds__(RED_THREADS) void k_primal(float* __restrict__ x,
                                                         const float* __restrict__ ga,
                                                         const float* __restrict__ gb,
                                                         const float* __restrict__ gc,
                                                         float* __restrict__ xhat, int64_t n,
+                                                        const float* __restrict__ truth,
                                                         double* partial) {
-  __shared__ double sh[2 * RED_THREADS / 64];
-  double acc[2] = {0.0, 0.0};
+  __shared__ double sh[3 * RED_THREADS / 64];
+  double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -68,23 +72,33 @@ __global__ __launch_bounds__(RED_THREADS) void k_primal(float* __restrict__ x,
     const double d = (double)prev - (double)xn;
     acc[0] += d * d;
     acc[1] += (double)xn * xn;
+    if (truth) {
+      const double e = (double)truth[i] - (double)xn;
+      acc[2] += e * e;
+    }
   }
-  block_sum<2>(acc, sh);
+  block_sum<3>(acc, sh);
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = acc[0];
     partial[gridDim.x + blockIdx.x] = acc[1];
+    partial[2 * gridDim.x + blockIdx.x] = acc[2];
   }
 }
 
+// out[v] = sum of partial vector v, for v < nvec; vector 2 goes to out2
 __global__ __launch_bounds__(RED_THREADS) void k_sum_partials(const double* __restrict__ partial,
                                                               int npart, int nvec,
-                                                              double* __restrict__ out) {
+                                                              double* __restrict__ out,
+                                                              double* __restrict__ out2) {
   __shared__ double sh[RED_THREADS / 64];
   for (int v = 0; v < nvec; ++v) {
     double s[1] = {0.0};
     for (int i = threadIdx.x; i < npart; i += blockDim.x) s[0] += partial[v * npart + i];
     block_sum<1>(s, sh);
-    if (threadIdx.x == 0) out[v] = s[0];
+    if (threadIdx.x == 0) {
+      if (v == 2) *out2 = s[0];
+      else out[v] = s[0];
+    }
   }
 }
 
@@ -96,13 +110,14 @@ unsigned red_grid(int64_t n) {
 }  // namespace
 
 void primal_update(fhs_ctx* ctx, float* x, const float* ga, const float* gb, const float* gc,
-                   float* xhat, int64_t n, double* d_out2) {
+                   float* xhat, int64_t n, double* d_out2, const float* truth, double* d_truth) {
   const unsigned G = red_grid(n);
-  double* part = (double*)ctx->ws.get("primal_partial", 2 * G * sizeof(double));
+  double* part = (double*)ctx->ws.get("primal_partial", 3 * G * sizeof(double));
   Scope sc(ctx, "primal");
-  k_primal<<<G, RED_THREADS, 0, ctx->stream>>>(x, ga, gb, gc, xhat, n, part);
+  k_primal<<<G, RED_THREADS, 0, ctx->stream>>>(x, ga, gb, gc, xhat, n, truth, part);
   check_launch();
-  k_sum_partials<<<1, RED_THREADS, 0, ctx->stream>>>(part, G, 2, d_out2);
+  const bool t = truth && d_truth;
+  k_sum_partials<<<1, RED_THREADS, 0, ctx->stream>>>(part, G, t ? 3 : 2, d_out2, d_truth);
   check_launch();
 }
 
@@ -135,7 +150,7 @@ int fhs_update_primal(fhs_ctx* ctx, float* xsol, const float* g_a, const float* 
   return guard(ctx, [&] {
     FHS_CHECK(xsol && g_a && xhat && n > 0, "update_primal: bad argument");
     double* d = (double*)ctx->ws.get("primal_out", 2 * sizeof(double));
-    primal_update(ctx, xsol, g_a, g_b, g_c, xhat, n, d);
+    primal_update(ctx, xsol, g_a, g_b, g_c, xhat, n, d, nullptr, nullptr);
     double h[2];
     FHS_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     FHS_HIP(hipStreamSynchronize(ctx->stream));

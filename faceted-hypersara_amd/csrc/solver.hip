@@ -55,8 +55,9 @@ struct fhs_pdfb {
   float sigma00 = 0.f, sigma11 = 0.f;
   int fb_max = 20, fb_min = 1, dr = 0;
   double fb_eps = 1e-8;
-  double* d_scal = nullptr;   // [4]
-  double* h_scal = nullptr;   // pinned [4]
+  double* d_scal = nullptr;   // [5]: the four stopping-rule sums, then sum (truth - x)^2
+  double* h_scal = nullptr;   // pinned [5]
+  const float* truth = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
@@ -74,7 +75,7 @@ void run_iteration(fhs_pdfb* P) {
   fhs_ctx* c = &P->own;
   fhs_ctx* p = P->concurrent ? &P->side : c;
   const auto& b = P->b;
-  primal_update(c, b.x, b.g1, b.g2, b.g0, b.xhat, P->N * P->L, P->d_scal);
+  primal_update(c, b.x, b.g1, b.g2, b.g0, b.xhat, P->N * P->L, P->d_scal, P->truth, P->d_scal + 4);
   if (P->concurrent) {
     FHS_HIP(hipEventRecord(P->ev_fork, c->stream));
     FHS_HIP(hipStreamWaitEvent(p->stream, P->ev_fork, 0));
@@ -141,8 +142,10 @@ int fhs_pdfb_create(fhs_ctx* ctx, fhs_data* D, fhs_sara* S, fhs_lowrank* R,
     P->Nx = b.Nx;
     P->N = (int64_t)b.Ny * b.Nx;
     if (R) FHS_CHECK(lowrank_channels(R) == P->L, "pdfb_create: low-rank prior has a different L");
-    FHS_HIP(hipMalloc(&P->d_scal, 4 * sizeof(double)));
-    FHS_HIP(hipHostMalloc(&P->h_scal, 4 * sizeof(double), hipHostMallocDefault));
+    FHS_HIP(hipMalloc(&P->d_scal, 5 * sizeof(double)));
+    FHS_HIP(hipMemset(P->d_scal, 0, 5 * sizeof(double)));
+    FHS_HIP(hipHostMalloc(&P->h_scal, 5 * sizeof(double), hipHostMallocDefault));
+    for (int k = 0; k < 5; ++k) P->h_scal[k] = 0.0;
     *out = P;
   });
 }
@@ -239,7 +242,7 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
         FHS_HIP(hipGraphLaunch(P->exec, c->stream));
       else
         run_iteration(P);
-      FHS_HIP(hipMemcpyAsync(P->h_scal, P->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      FHS_HIP(hipMemcpyAsync(P->h_scal, P->d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       FHS_HIP(hipStreamSynchronize(c->stream));
       if (scalars)
         for (int k = 0; k < 4; ++k) scalars[4 * it + k] = P->h_scal[k];
@@ -251,6 +254,24 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
     }
   });
   return st;
+}
+
+int fhs_pdfb_set_truth(fhs_pdfb* P, const float* X0) {
+  if (!P) return FHS_ERR_ARG;
+  return guard(P->ctx, [&] {
+    FHS_HIP(hipStreamSynchronize(P->own.stream));
+    if (X0 != P->truth) drop_graph(P);   // the pointer is a kernel argument of the captured graph
+    P->truth = X0;
+  });
+}
+
+int fhs_pdfb_truth_error(fhs_pdfb* P, double* err2) {
+  if (!P || !err2) return FHS_ERR_ARG;
+  return guard(P->ctx, [&] {
+    FHS_CHECK(P->truth, "pdfb_truth_error: no reference cube set (fhs_pdfb_set_truth)");
+    FHS_CHECK(P->iters > 0, "pdfb_truth_error: no iteration has run");
+    *err2 = P->h_scal[4];
+  });
 }
 
 int fhs_pdfb_norm_res(fhs_pdfb* P, double* norm_res) {

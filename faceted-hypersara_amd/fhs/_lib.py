@@ -88,6 +88,8 @@ PROTOTYPES = {
     'fhs_pdfb_set_concurrent': [c_void_p, c_int],
     'fhs_pdfb_iterate': [c_void_p, c_int, c_double_p],
     'fhs_pdfb_norm_res': [c_void_p, c_double_p],
+    'fhs_pdfb_set_truth': [c_void_p, c_void_p],
+    'fhs_pdfb_truth_error': [c_void_p, c_double_p],
     'fhs_pdfb_stream': [c_void_p, C.POINTER(c_void_p)],
     'fhs_data_forward': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_data_adjoint': [c_void_p, c_void_p, c_void_p, c_void_p, c_float_p, c_void_p],

@@ -239,6 +239,21 @@ class PdfbEngine:
                        'pdfb_iterate')
         return out.reshape(n, 4)
 
+    def set_truth(self, X0):
+        """Reference cube (L, Nx, Ny) float32 on the GPU, or None: the primal
+        update then also sums ||X0 - x||^2 (see ``truth_error``)."""
+        if X0 is not None:
+            X0 = X0.detach().to(device=self.ctx.device, dtype=torch.float32).reshape(
+                self.meas.L, self.meas.Nx, self.meas.Ny).contiguous()
+        self._truth = X0          # the engine keeps the pointer
+        self.ctx.check(lib().fhs_pdfb_set_truth(self.h, None if X0 is None else _ptr(X0)), 'pdfb_set_truth')
+
+    def truth_error(self):
+        """||X0 - x||^2 after the last iteration (FP64, no GPU call)."""
+        out = C.c_double()
+        self.ctx.check(lib().fhs_pdfb_truth_error(self.h, C.byref(out)), 'pdfb_truth_error')
+        return out.value
+
     def norm_res(self):
         out = np.zeros(self.meas.D.nblocks)
         self.ctx.check(lib().fhs_pdfb_norm_res(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), 'pdfb_norm_res')
@@ -346,9 +361,10 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
             'converged': False}
     track = X0 is not None and (callback is not None or prm.get('track_snr', False))
     if track:
+        # the primal kernel sums ||X0 - x||^2 with its own reductions
         X0d = X0 if isinstance(X0, torch.Tensor) else _cube(X0)
-        X0d = X0d.to(ctx.device).double()
-        nX0 = torch.linalg.vector_norm(X0d)
+        eng.set_truth(X0d)
+        nX0 = float(torch.linalg.vector_norm(eng._truth.double()))
     t0 = time.perf_counter()
     t = 0
     for t in range(1, max_iter + 1):
@@ -360,7 +376,8 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         info['norm_eps'].append(neps)
         info['time'].append(time.perf_counter() - t0)
         if track:
-            info['snr'].append(20 * np.log10((nX0 / torch.linalg.vector_norm(X0d - eng.x.double())).item()))
+            e2 = eng.truth_error()
+            info['snr'].append(20 * np.log10(nX0 / sqrt(e2)) if e2 > 0 else float('inf'))
         if callback is not None and callback(t, eng, info) is False:
             break
         k = t - last_step

@@ -305,6 +305,15 @@ int fhs_pdfb_set_concurrent(fhs_pdfb* p, int enable);
 /* runs n_iter iterations; scalars: host double[4*n_iter] or NULL:
  * sum (x_prev - x)^2, sum x^2, sum_b norm_res_b^2, sum_b eps_b^2 */
 int fhs_pdfb_iterate(fhs_pdfb* p, int n_iter, double* scalars);
+/* Reference cube X0 (device float[L][Nx][Ny], NULL = none) whose distance
+ * to x the primal update sums in FP64 each iteration, fused with its other
+ * reductions. This replaces the per-iteration SNR evaluation
+ * `snr = 20*log10(norm(X0(:))/norm(X0(:)-xsol(:)))` of the solvers' logging
+ * (src/hs/hyperSARA.m:549, src/fhs/facetHyperSARA.m:864) without a second pass over
+ * the cube or a second synchronisation. X0 must stay valid while set. */
+int fhs_pdfb_set_truth(fhs_pdfb* p, const float* X0);
+/* last iteration's sum (X0 - x)^2 (host copy, no GPU call) */
+int fhs_pdfb_truth_error(fhs_pdfb* p, double* err2);
 /* last iteration's per-block residual norms ||G_b A x - y_b||: host double[nblocks] */
 int fhs_pdfb_norm_res(fhs_pdfb* p, double* norm_res);
 int fhs_pdfb_stream(fhs_pdfb* p, void** stream);

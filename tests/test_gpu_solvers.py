@@ -110,3 +110,25 @@ def test_concurrent_streams_match_sequential(ctx):
     xa, _ = solvers.hyper_sara(meas, p, max_total_iter=20, concurrent=True)
     xb, _ = solvers.hyper_sara(meas, p, max_total_iter=20, concurrent=False)
     assert np.array_equal(xa, xb)
+
+
+def test_fused_snr_tracking(ctx):
+    """The SNR summed inside the primal update equals the SNR of the cube,
+    computed in FP64 on the host, at every iteration (graph replay included),
+    and following it does not change the iterates."""
+    from fhs import solvers, synth
+    prob = synth.make_problem(64, 64, 4, 4000, 2, seed=19, n_sources=30)
+    meas = solvers.Measurement(ctx, prob, precond=True)
+    son, nu2 = meas.op_norm(False), meas.op_norm(True)
+    p = {'squared_operator_norm': son, 'nu2': nu2, 'track_snr': True}
+    host = []
+
+    def cb(t, eng, info):
+        host.append(snr(prob.X0, solvers._uncube(eng.x)))
+
+    x_t, info = solvers.hyper_sara(meas, p, X0=prob.X0, max_total_iter=15, callback=cb)
+    assert len(info['snr']) == len(host) == 15
+    # X0 is rounded to FP32 once; the sums are FP64
+    assert np.allclose(info['snr'], host, rtol=0, atol=1e-4), (info['snr'], host)
+    x_u, _ = solvers.hyper_sara(meas, {'squared_operator_norm': son, 'nu2': nu2}, max_total_iter=15)
+    assert np.array_equal(x_t, x_u)
