@@ -45,12 +45,15 @@ struct fhs_data {
   float2* Fx = nullptr;                // [L][No]
   float2* g2 = nullptr;                // [L][No]
   float2* r2 = nullptr;                // [Mtot]
+  int fb_split = 1;                    // workgroups per block in the FB projection
+  unsigned* fb_sync = nullptr;         // device [2 nblocks + 1]: arrival counts, generations, error flag
+  double* fb_part = nullptr;           // device [nblocks][2][FB_SPLIT_MAX][FB_PART], then [nblocks][FB_SPLIT_MAX]
 };
 
 namespace fhs {
 namespace {
 
-constexpr int FB_THREADS = 1024;
+constexpr int FB_THREADS = 512;
 constexpr int NXCD = 8;   // MI355X: 8 XCDs, each with its own L2
 
 // Workgroups of a launch are dispatched to the XCDs round-robin (linear id
@@ -114,31 +117,178 @@ __global__ __launch_bounds__(256) void k_data_forward_dr(const GDev* __restrict_
   gx_old[i] = gx;
 }
 
-// One workgroup per data block: forward-backward projection onto
-// B(y, eps) in the metric diag(pU), then the dual update and the residual.
-// z <- P_B(z - mu pU (z - w)), w = v2/pU + r2, mu = 1/max(pU), warm start
-// z = proj; stop after min_it once ||dz|| <= tol ||z|| or at max_it.
-// w is formed once and kept in the v2 buffer during the iterations (28 B
-// read + 8 B written per row and iteration instead of 44 B); the dual
-// update v2 + pU (r2 - z) is then pU (w - z).
+// ---- forward-backward projection onto the data ellipsoid ------------------
+// Per data block: z <- P_B(z - mu pU (z - w)), w = v2/pU + r2, mu = 1/max(pU),
+// warm start z = proj, stop after min_it once ||dz|| <= tol ||z|| or at
+// max_it; then the dual update v2 = pU (w - z) and the residual ||gx - y||.
+//
+// A block is split over S workgroups (rows in S contiguous parts), so the
+// whole GPU works on the projection instead of one CU per block.  Each
+// thread keeps FB_ROWS of its rows (w, z, y, pU) in registers for the whole
+// call: the FB iterations then read no memory for them, only rows beyond
+// FB_ROWS * FB_THREADS per workgroup are streamed (w kept in the v2 buffer,
+// z in proj).  The block-wide sums every iteration needs are combined
+// across the S workgroups by fb_combine.  The launch never has more
+// workgroups than can be resident at once (fb_split), so every wait ends.
+constexpr int FB_ROWS = 24;                 // rows per thread held in registers
+constexpr int FB_SPLIT_MAX = 16;            // workgroups per block
+constexpr int FB_PART = 4;                  // doubles per workgroup partial
+constexpr unsigned FB_SPIN_LIMIT = 1u << 22;
+// dynamic LDS per workgroup: y and pU of the resident rows
+constexpr size_t FB_LDS = (size_t)FB_ROWS * FB_THREADS * (sizeof(float2) + sizeof(float));
+
+struct FbSync {
+  unsigned* cnt;    // [nblocks] arrivals in the current round
+  unsigned* gen;    // [nblocks] completed rounds
+  double* part;     // [nblocks][2][FB_SPLIT_MAX][FB_PART] partials, by round parity
+  unsigned* err;    // set when a wait gave up
+  double* res;      // [nblocks][FB_SPLIT_MAX] residual sums of the parts
+};
+
+// Combine NV per-thread values over the S workgroups of block blk (sum, or
+// max with MAXOP).  Thread 0 publishes the workgroup's total in the slot of
+// this round's parity and counts its arrival with an agent-scope atomic; the
+// last arrival resets the count and advances the block's generation, the
+// others wait for that.  Every workgroup then combines the S partials in
+// slot order, so all of them hold bit-identical totals.  A workgroup reuses
+// a slot two rounds later, after every workgroup has read it.
+// All shared words are accessed with agent-scope atomics, which the GPU
+// performs at its device-coherent level (past the per-XCD L2s), and each
+// step waits for the previous one's completion (s_waitcnt 0) before it is
+// issued; so no L2 writeback or invalidation is needed, which would also
+// flush the dirty lines of kernels running beside this one.
+// The wait is bounded: after FB_SPIN_LIMIT polls it sets *err and goes on
+// (the results are then wrong; the host reports it).
+template <bool MAXOP, int NV>
+__device__ void fb_combine(const FbSync& fs, int blk, int part, int S, unsigned& gen, int& round,
+                           double (&v)[NV], double* sh, double* bc) {
+  if (MAXOP) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double m = v[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+      v[k] = m;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < NV; ++k) sh[k * (FB_THREADS / 64) + (threadIdx.x >> 6)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double m = sh[k * (FB_THREADS / 64)];
+      for (int w = 1; w < FB_THREADS / 64; ++w) m = fmax(m, sh[k * (FB_THREADS / 64) + w]);
+      v[k] = m;
+    }
+    __syncthreads();
+  } else {
+    block_sum<NV>(v, sh);
+  }
+  if (S > 1) {
+    if (threadIdx.x == 0) {
+      double* slot = fs.part + ((size_t)blk * 2 + (round & 1)) * (FB_SPLIT_MAX * FB_PART);
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        __hip_atomic_store(slot + part * FB_PART + k, v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);   // the partials are stored before the arrival counts
+      const unsigned old = __hip_atomic_fetch_add(fs.cnt + blk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old == (unsigned)S - 1) {
+        __hip_atomic_store(fs.cnt + blk, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);   // the reset lands before the generation advances
+        __hip_atomic_store(fs.gen + blk, gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        unsigned spins = 0;
+        while (__hip_atomic_load(fs.gen + blk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins >= FB_SPIN_LIMIT) {
+            __hip_atomic_store(fs.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      ++gen;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        double t = __hip_atomic_load(slot + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int w = 1; w < S; ++w) {
+          const double u = __hip_atomic_load(slot + w * FB_PART + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          t = MAXOP ? fmax(t, u) : t + u;
+        }
+        bc[k] = t;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = bc[k];
+    __syncthreads();
+  }
+  ++round;
+}
+
 __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
     const int64_t* __restrict__ blk_start, const double* __restrict__ eps_arr,
     float2* __restrict__ v2, const float2* __restrict__ y, const float* __restrict__ pU,
     const float2* __restrict__ r2, float2* __restrict__ z, const float2* __restrict__ gx,
-    int max_it, int min_it, double tol, double* __restrict__ norm_res) {
+    int max_it, int min_it, double tol, int S, FbSync fs) {
   __shared__ double sh[4 * FB_THREADS / 64];
-  __shared__ float shm[FB_THREADS / 64];
-  const int64_t a = blk_start[blockIdx.x], b = blk_start[blockIdx.x + 1];
-  const double eps = eps_arr[blockIdx.x];
-  // max(pU) -> step
-  float pmax = 0.f;
-#pragma unroll 4
-  for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) pmax = fmaxf(pmax, pU ? pU[i] : 1.f);
-  pmax = wave_max(pmax);
-  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = pmax;
-  __syncthreads();
-  pmax = 0.f;
-  for (int w = 0; w < FB_THREADS / 64; ++w) pmax = fmaxf(pmax, shm[w]);
+  __shared__ double bc[4];
+  // y [FB_ROWS][FB_THREADS] then pU [FB_ROWS][FB_THREADS] of the resident
+  // rows; every thread reads back only its own slots, so no barrier is needed
+  extern __shared__ float2 fb_y[];
+  const int blk = blockIdx.x / S, part = blockIdx.x % S;
+  const int64_t a0 = blk_start[blk], b0 = blk_start[blk + 1];
+  int64_t cs = (b0 - a0 + S - 1) / S;
+  cs = (cs + 63) & ~(int64_t)63;
+  const int64_t a = a0 + part * cs < b0 ? a0 + part * cs : b0;
+  const int64_t b = a + cs < b0 ? a + cs : b0;
+  const int64_t tail = a + (int64_t)FB_ROWS * FB_THREADS;   // first streamed row
+  const int nloc = (int)(b - a);
+  const double eps = eps_arr[blk];
+  unsigned gen = 0;
+  int round = 0;
+  if (S > 1 && threadIdx.x == 0) gen = __hip_atomic_load(fs.gen + blk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  auto live = [&](int k) { return (int)threadIdx.x + k * FB_THREADS < nloc; };
+  // Resident row k of this thread is a + threadIdx.x + k FB_THREADS.  Loads
+  // use indices clamped into the part (or the block's first row for an empty
+  // part), so they are unconditional and batch; rows past the part only
+  // feed masked-out terms and are never stored.
+  const int64_t base = nloc > 0 ? a : a0;
+  const unsigned last = nloc > 0 ? (unsigned)nloc - 1u : 0u;
+  // unsigned min: the index is provably below FB_ROWS * FB_THREADS, so the
+  // loads take a scalar base and a 32-bit offset instead of a 64-bit address per row
+  auto ridx = [&](int k) { return min(threadIdx.x + (unsigned)(k * FB_THREADS), last); };
+  float2 rw[FB_ROWS], rz[FB_ROWS];
+  float2* ry = fb_y + threadIdx.x;                                    // ry[k * FB_THREADS]
+  float* rp = (float*)(fb_y + FB_ROWS * FB_THREADS) + threadIdx.x;   // rp[k * FB_THREADS]
+  const float* pUa = pU ? pU + base : nullptr;
+  const float2 *v2a = v2 + base, *r2a = r2 + base, *za = z + base, *ya = y + base;
+  // two waves of loads (pU, v2, r2 -> w, then z, y) bound the live registers
+#pragma unroll
+  for (int k = 0; k < FB_ROWS; ++k) {
+    const int i = ridx(k);
+    const float p = pUa ? pUa[i] : 1.f;
+    rp[k * FB_THREADS] = p;
+    const float2 vv = v2a[i], r = r2a[i];
+    rw[k] = make_float2(vv.x / p + r.x, vv.y / p + r.y);   // w = v2 / pU + r2
+  }
+#pragma unroll
+  for (int k = 0; k < FB_ROWS; ++k) {
+    const int i = ridx(k);
+    rz[k] = za[i];
+    ry[k * FB_THREADS] = ya[i];
+  }
+  // max(pU) over the block -> step (a clamped duplicate does not change a max)
+  double pmv[1] = {0.0};
+  if (nloc > 0) {
+#pragma unroll
+    for (int k = 0; k < FB_ROWS; ++k) pmv[0] = fmax(pmv[0], (double)rp[k * FB_THREADS]);
+  }
+  for (int64_t i = tail + threadIdx.x; i < b; i += FB_THREADS) pmv[0] = fmax(pmv[0], pU ? (double)pU[i] : 1.0);
+  fb_combine<true, 1>(fs, blk, part, S, gen, round, pmv, sh, bc);
+  const float pmax = (float)pmv[0];
   const float pm = pmax > 0.f ? pmax : 1.f;
   const float inv_pm = 1.f / pm;
   // a = pU / max pU as one multiply; exactly 1 where pU is maximal
@@ -146,57 +296,114 @@ __global__ __launch_bounds__(FB_THREADS) void k_fb_dual(
   // gradient step z - mu pU (z - w), mu = 1/max pU, as (1-a) z + a w with
   // a = pU / max pU (a == 1 exactly where pU is maximal, so a constant pU
   // reproduces w bit for bit and the iteration stops at its fixed point).
-  // Pass 1: w into v2, first gradient-step distance to y.
+  // Pass 1: first gradient-step distance to y.
   double s1[1] = {0.0};
-#pragma unroll 4
-  for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
+#pragma unroll
+  for (int k = 0; k < FB_ROWS; ++k) {
+    const float2 w = rw[k], yy = ry[k * FB_THREADS];
+    const float aa = step(rp[k * FB_THREADS]);
+    const float2 g = make_float2((1.f - aa) * rz[k].x + aa * w.x, (1.f - aa) * rz[k].y + aa * w.y);
+    const double dx = (double)g.x - yy.x, dy = (double)g.y - yy.y;
+    s1[0] += live(k) ? dx * dx + dy * dy : 0.0;
+  }
+  for (int64_t i = tail + threadIdx.x; i < b; i += FB_THREADS) {
     const float p = pU ? pU[i] : 1.f;
-    const float2 vv = v2[i], rr = r2[i], zz = z[i], yy = y[i];
-    const float2 w = make_float2(vv.x / p + rr.x, vv.y / p + rr.y);
+    const float2 vv = v2[i], r = r2[i], zz = z[i], yy = y[i];
+    const float2 w = make_float2(vv.x / p + r.x, vv.y / p + r.y);
     v2[i] = w;
     const float aa = step(p);
     const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
     const double dx = (double)g.x - yy.x, dy = (double)g.y - yy.y;
     s1[0] += dx * dx + dy * dy;
   }
-  block_sum<1>(s1, sh);
+  fb_combine<false, 1>(fs, blk, part, S, gen, round, s1, sh, bc);
   double s_cur = s1[0];
   for (int it = 1; it <= max_it; ++it) {
     const double nrm = sqrt(s_cur);
     const float scale = nrm > 0.0 ? (float)fmin(eps / nrm, 1.0) : 1.f;
     double acc[3] = {0.0, 0.0, 0.0};  // ||dz||^2, ||z||^2, next ||g - y||^2
-#pragma unroll 4
-    for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
-      const float p = pU ? pU[i] : 1.f;
-      const float2 w = v2[i], zz = z[i], yy = y[i];
+    auto upd = [&](float p, float2 w, float2& zz, float2 yy, bool ok) {
       const float aa = step(p);
       const float2 g = make_float2((1.f - aa) * zz.x + aa * w.x, (1.f - aa) * zz.y + aa * w.y);
       const float2 zn = make_float2(yy.x + (g.x - yy.x) * scale, yy.y + (g.y - yy.y) * scale);
-      z[i] = zn;
       const double ddx = (double)zn.x - zz.x, ddy = (double)zn.y - zz.y;
-      acc[0] += ddx * ddx + ddy * ddy;
-      acc[1] += (double)zn.x * zn.x + (double)zn.y * zn.y;
+      const double zx = zn.x, zy = zn.y;
       const float2 g2 = make_float2((1.f - aa) * zn.x + aa * w.x, (1.f - aa) * zn.y + aa * w.y);
       const double ex = (double)g2.x - yy.x, ey = (double)g2.y - yy.y;
-      acc[2] += ex * ex + ey * ey;
+      acc[0] += ok ? ddx * ddx + ddy * ddy : 0.0;
+      acc[1] += ok ? zx * zx + zy * zy : 0.0;
+      acc[2] += ok ? ex * ex + ey * ey : 0.0;
+      zz = zn;
+    };
+#pragma unroll
+    for (int k = 0; k < FB_ROWS; ++k) upd(rp[k * FB_THREADS], rw[k], rz[k], ry[k * FB_THREADS], live(k));
+    for (int64_t i = tail + threadIdx.x; i < b; i += FB_THREADS) {
+      float2 zz = z[i];
+      upd(pU ? pU[i] : 1.f, v2[i], zz, y[i], true);
+      z[i] = zz;
     }
-    block_sum<3>(acc, sh);
+    fb_combine<false, 3>(fs, blk, part, S, gen, round, acc, sh, bc);
     const double crit = acc[1] > 0.0 ? sqrt(acc[0] / acc[1]) : 0.0;
     if (it >= min_it && crit <= tol) break;
     s_cur = acc[2];
   }
-  // dual update v2 = pU (w - z) and residual ||G x - y||
+  // dual update v2 = pU (w - z), z kept as the next warm start, residual ||gx - y||
   double res[1] = {0.0};
-#pragma unroll 4
-  for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
+  const float2* gxa = gx + base;
+  float2 *v2out = v2 + base, *zout = z + base;
+#pragma unroll
+  for (int k = 0; k < FB_ROWS; ++k) {
+    const float2 gg = gxa[ridx(k)], yy = ry[k * FB_THREADS];
+    const double ex = (double)gg.x - yy.x, ey = (double)gg.y - yy.y;
+    res[0] += live(k) ? ex * ex + ey * ey : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < FB_ROWS; ++k) {
+    if (live(k)) {
+      const unsigned i = threadIdx.x + (unsigned)(k * FB_THREADS);
+      const float p = rp[k * FB_THREADS];
+      v2out[i] = make_float2(p * (rw[k].x - rz[k].x), p * (rw[k].y - rz[k].y));
+      zout[i] = rz[k];
+    }
+  }
+  for (int64_t i = tail + threadIdx.x; i < b; i += FB_THREADS) {
     const float p = pU ? pU[i] : 1.f;
     const float2 w = v2[i], zz = z[i], yy = y[i], gg = gx[i];
     v2[i] = make_float2(p * (w.x - zz.x), p * (w.y - zz.y));
     const double ex = (double)gg.x - yy.x, ey = (double)gg.y - yy.y;
     res[0] += ex * ex + ey * ey;
   }
+  // the parts' residual sums are added by k_sum_blocks (no exchange needed)
   block_sum<1>(res, sh);
-  if (threadIdx.x == 0) norm_res[blockIdx.x] = res[0];
+  if (threadIdx.x == 0) fs.res[(size_t)blk * FB_SPLIT_MAX + part] = res[0];
+}
+
+FbSync fb_sync(const fhs_data* D) {
+  FbSync f;
+  f.cnt = D->fb_sync;
+  f.gen = D->fb_sync + D->nblocks;
+  f.err = D->fb_sync + 2 * D->nblocks;
+  f.part = D->fb_part;
+  f.res = D->fb_part + (size_t)D->nblocks * 2 * FB_SPLIT_MAX * FB_PART;
+  return f;
+}
+
+// Workgroups per block for the FB projection: enough that every row of the
+// largest block is held in registers, at most FB_SPLIT_MAX, and never more
+// workgroups in total than the GPU holds at once (the combine waits for all
+// workgroups of a block, so they must be able to be resident together).
+int fb_split_for(const fhs_data* D, int device) {
+  int64_t maxb = 0;
+  for (int b = 0; b < D->nblocks; ++b) maxb = std::max<int64_t>(maxb, D->blk_start[b + 1] - D->blk_start[b]);
+  const int64_t per = (int64_t)FB_ROWS * FB_THREADS;
+  int S = (int)std::min<int64_t>(FB_SPLIT_MAX, std::max<int64_t>(1, (maxb + per - 1) / per));
+  int occ = 0, cus = 0;
+  FHS_HIP(hipFuncSetAttribute((const void*)k_fb_dual, hipFuncAttributeMaxDynamicSharedMemorySize, (int)FB_LDS));
+  FHS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fb_dual, FB_THREADS, FB_LDS));
+  FHS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int64_t cap = (int64_t)occ * cus;
+  while (S > 1 && (int64_t)D->nblocks * S > cap) --S;
+  return S;
 }
 
 // DR dual step: v2 <- sc(v2 + r2 - y, eps) (closed-form l2 prox), residual
@@ -399,15 +606,27 @@ void launch_data_forward(fhs_ctx* ctx, const fhs_data* D, float2* gx_old, unsign
 }
 
 
-__global__ void k_sum_blocks(const double* __restrict__ norm_res, const double* __restrict__ eps,
-                             int nb, double* __restrict__ out2) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+// Block residuals from the FB parts (fb_res, S parts per block, added in
+// part order; NULL when the residuals are already per block), then their
+// sum and that of the squared radii into out2 (may be NULL).  A NaN residual
+// sum reports a failed FB workgroup exchange (*fb_err set).
+__global__ void k_sum_blocks(double* __restrict__ norm_res, const double* __restrict__ fb_res, int S,
+                             const double* __restrict__ eps, int nb, const unsigned* __restrict__ fb_err,
+                             double* __restrict__ out2) {
+  if (fb_res)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      double r = 0.0;
+      for (int p = 0; p < S; ++p) r += fb_res[(size_t)b * FB_SPLIT_MAX + p];
+      norm_res[b] = r;
+    }
+  __syncthreads();
+  if (out2 && threadIdx.x == 0) {
     double s = 0.0, e = 0.0;
     for (int i = 0; i < nb; ++i) {
       s += norm_res[i];
       e += eps[i] * eps[i];
     }
-    out2[0] = s;
+    out2[0] = *fb_err ? __builtin_nan("") : s;
     out2[1] = e;
   }
 }
@@ -446,8 +665,9 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
       k_l2_dual<<<D->nblocks, FB_THREADS, 0, ctx->stream>>>(D->d_blk_start, D->d_eps, v2, y, D->r2, gx_old,
                                                             D->d_norm_res);
     else
-      k_fb_dual<<<D->nblocks, FB_THREADS, 0, ctx->stream>>>(D->d_blk_start, D->d_eps, v2, y, pU, D->r2,
-                                                            proj, gx_old, max_it, min_it, tol, D->d_norm_res);
+      k_fb_dual<<<D->nblocks * D->fb_split, FB_THREADS, FB_LDS, ctx->stream>>>(
+          D->d_blk_start, D->d_eps, v2, y, pU, D->r2, proj, gx_old, max_it, min_it, tol, D->fb_split,
+          fb_sync(D));
     check_launch();
   }
   {
@@ -455,8 +675,9 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
     launch_data_adjoint(ctx, D, v2, lam, dr);
   }
   nufft_adjoint(ctx, D->A, D->g2, L, D->d_sigma22, Ftx);
-  if (d_out2) {
-    k_sum_blocks<<<1, 64, 0, ctx->stream>>>(D->d_norm_res, D->d_eps, D->nblocks, d_out2);
+  if (d_out2 || !dr) {
+    k_sum_blocks<<<1, 256, 0, ctx->stream>>>(D->d_norm_res, dr ? nullptr : fb_sync(D).res, D->fb_split, D->d_eps,
+                                             D->nblocks, D->fb_sync + 2 * D->nblocks, d_out2);
     check_launch();
   }
 }
@@ -511,9 +732,12 @@ void data_adjoint(fhs_ctx* ctx, fhs_data* D, const float2* v, const float* lam, 
 
 void data_get_norm_res(fhs_ctx* ctx, fhs_data* D, double* h_out) {
   std::vector<double> nr(D->nblocks);
+  unsigned err = 0;
   FHS_HIP(hipMemcpyAsync(nr.data(), D->d_norm_res, D->nblocks * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
+  FHS_HIP(hipMemcpyAsync(&err, D->fb_sync + 2 * D->nblocks, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
   FHS_HIP(hipStreamSynchronize(ctx->stream));
+  FHS_CHECK(err == 0, "data step: a workgroup of the ellipsoid projection timed out waiting for its block");
   for (int b = 0; b < D->nblocks; ++b) h_out[b] = std::sqrt(nr[b]);
 }
 
@@ -585,6 +809,10 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
     FHS_HIP(hipMalloc(&D->Fx, (size_t)L * D->No * sizeof(float2)));
     FHS_HIP(hipMalloc(&D->g2, (size_t)L * D->No * sizeof(float2)));
     FHS_HIP(hipMalloc(&D->r2, (size_t)D->Mtot * sizeof(float2)));
+    FHS_HIP(hipMalloc(&D->fb_sync, (2 * (size_t)nblocks + 1) * sizeof(unsigned)));
+    FHS_HIP(hipMemset(D->fb_sync, 0, (2 * (size_t)nblocks + 1) * sizeof(unsigned)));
+    FHS_HIP(hipMalloc(&D->fb_part, (size_t)nblocks * (2 * FB_SPLIT_MAX * FB_PART + FB_SPLIT_MAX) * sizeof(double)));
+    D->fb_split = fb_split_for(D, ctx->device);
     *out = D;
   });
 }
@@ -593,7 +821,7 @@ int fhs_data_destroy(fhs_data* D) {
   if (!D) return FHS_OK;
   (void)hipSetDevice(D->device);
   void* ps[] = {D->gdev, D->d_row_off, D->d_blk_start, D->d_eps, D->d_norm_res,
-                D->d_sigma22, D->Fx, D->g2, D->r2};
+                D->d_sigma22, D->Fx, D->g2, D->r2, D->fb_sync, D->fb_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete D;

@@ -22,7 +22,7 @@ def f32(a):
         np.complex128 if np.iscomplexobj(a) else np.float64)
 
 
-def make(rng, Ny, Nx, L, M, nblk, precond):
+def make(rng, Ny, Nx, L, M, nblk, precond, cuts=None):
     ops, ys, pus, blocks = [], [], [], []
     for l in range(L):
         u = np.clip(rng.normal(0, np.pi / 4, M), -3.1, 3.1) * (0.8 + 0.05 * l)
@@ -33,18 +33,25 @@ def make(rng, Ny, Nx, L, M, nblk, precond):
         x0 = rng.random((Ny, Nx))
         ys.append(f32(op.G @ op.A(x0) + 0.1 * (rng.normal(size=M) + 1j * rng.normal(size=M))))
         pus.append(f32(uniform_precond_weights(u, v, op.Kty, op.Ktx) if precond else np.ones(M)))
-        b = np.linspace(0, M, nblk + 1).astype(int)
+        b = np.linspace(0, M, nblk + 1).astype(int) if cuts is None else np.asarray(cuts)
         blocks.append([(b[i], b[i + 1]) for i in range(nblk)])
     return ops, ys, pus, blocks
 
 
-@pytest.mark.parametrize('precond,dims', [(True, (64, 64, 3, 3000, 3)), (False, (32, 64, 2, 1500, 2)),
-                                          (True, (256, 256, 4, 30000, 4))])
-def test_update_dual_data_fidelity(ctx, precond, dims):
+# The ellipsoid projection splits a block over several workgroups once it
+# has more rows than one workgroup holds on chip (12,288): 60,000 rows take
+# 5 workgroups; 250,000 rows take 16, each also streaming rows beyond its
+# on-chip share; the uneven cut (59,900 + 100 rows) leaves workgroups of the
+# small block without rows.
+@pytest.mark.parametrize('precond,dims,cuts', [
+    (True, (64, 64, 3, 3000, 3), None), (False, (32, 64, 2, 1500, 2), None),
+    (True, (256, 256, 4, 30000, 4), None), (True, (64, 64, 2, 60000, 1), None),
+    (True, (64, 64, 2, 250000, 1), None), (True, (64, 64, 2, 60000, 2), (0, 59900, 60000))])
+def test_update_dual_data_fidelity(ctx, precond, dims, cuts):
     import fhs
     Ny, Nx, L, M, nblk = dims
     rng = np.random.default_rng(M)
-    ops, ys, pus, blocks = make(rng, Ny, Nx, L, M, nblk, precond)
+    ops, ys, pus, blocks = make(rng, Ny, Nx, L, M, nblk, precond, cuts)
     plan = fhs.NufftPlan(ctx, Ny, Nx, ops[0].Kty, ops[0].Ktx, ops[0].sy, ops[0].sx)
     Gs = [fhs.GMatrix.from_separable(ctx, op.anchor, op.cy_sep, op.cx_sep, op.Kty, op.Ktx) if l % 2 else
           fhs.GMatrix.from_window(ctx, op.anchor, op.coef, op.Kty, op.Ktx, op.Ky, op.Kx)
@@ -84,3 +91,31 @@ def test_update_dual_data_fidelity(ctx, precond, dims):
         assert rel(Ftxd.cpu().numpy().transpose(2, 1, 0), Ftx) < TOL, it
         assert rel(nrd, np.concatenate(nr)) < TOL, it
         assert abs(gnrd - gnr) / gnr < TOL and abs(nepsd - neps) / neps < 1e-12
+
+
+def test_split_projection_is_deterministic(ctx):
+    """A block split over several workgroups combines their sums in a fixed
+    order: repeating the step from the same state is bit-identical."""
+    import fhs
+    Ny, Nx, L, M = 64, 64, 2, 60000
+    rng = np.random.default_rng(7)
+    ops, ys, pus, blocks = make(rng, Ny, Nx, L, M, 1, True)
+    plan = fhs.NufftPlan(ctx, Ny, Nx, ops[0].Kty, ops[0].Ktx, ops[0].sy, ops[0].sx)
+    Gs = [fhs.GMatrix.from_separable(ctx, op.anchor, op.cy_sep, op.cx_sep, op.Kty, op.Ktx) for op in ops]
+    D = fhs.DataOperator(ctx, plan, Gs, [[M]] * L)
+    yd = torch.from_numpy(np.concatenate(ys).astype(np.complex64)).cuda()
+    pUd = torch.from_numpy(np.concatenate(pus).astype(np.float32)).cuda()
+    eps = np.array([0.05 * np.linalg.norm(ys[l]) for l in range(L)])
+    x = np.maximum(rng.normal(size=(L, Nx, Ny)), 0).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    sigma22 = np.array([0.3, 0.4], dtype=np.float32)
+    outs = []
+    for _ in range(2):
+        v2d, projd = D.zeros(), D.zeros()
+        gx_old = D.init_fx_old(torch.zeros_like(xd))
+        for it in range(3):
+            v2d, Ftxd, gx_old, projd, nrd, gnrd, nepsd = fhs.update_dual_data_fidelity(
+                v2d, yd, xd * (1 + 0.1 * it), gx_old, projd, plan, plan, D, None, pUd, eps, 20, 1, 1e-8, sigma22)
+        outs.append((v2d.cpu().numpy(), projd.cpu().numpy(), np.asarray(nrd)))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
