@@ -8,11 +8,14 @@
 //   g0 = sigma * w .* v0
 // Three launches:
 //   1. Gram Z'Z in FP64 (FP32 inputs widened, so every product is exact).
-//      Forming the Gram squares the condition number, which FP32
-//      accumulation cannot absorb (measured: 2e-4 relative error of the
-//      prox on near-low-rank cubes); FP64 keeps the small singular values,
-//      those near the threshold, accurate.  One partial 16 x 16 Gram per
-//      1024-pixel chunk; HBM-bound (16 L P bytes per facet).
+//      Forming the Gram squares the condition number, and the prox of a
+//      near-low-rank cube amplifies Gram errors by ~10^3: FP32 accumulation
+//      (2e-4 relative error of the prox, measured) and also BF16 hi/mid/lo
+//      splits on the matrix cores, whose 16-product sums are rounded to FP32
+//      (Gram error 4e-9, prox error 1.1e-5 > 1e-5), fail; FP64 sums keep the
+//      small singular values, those near the threshold, accurate.  One
+//      partial 16 x 16 Gram per 1024-pixel chunk; HBM-bound (16 L P bytes
+//      per facet).
 //   2. Per facet (one wavefront): FP64 sum of the partials, cyclic Jacobi
 //      eigen-decomposition warm-started from the previous iteration's
 //      eigenvectors (the Gram changes little between iterations, so one or
@@ -56,51 +59,91 @@ __device__ __forceinline__ float z_val(const FacetDev& F, int q, int L, int P, i
   return vv + w * xv;
 }
 
-// Partial Gram of one 1024-pixel chunk in FP64: 64-pixel tiles of Z are
-// staged in LDS (coalesced along pixels), thread t owns entry
-// (t / 16, t % 16) and accumulates z_p[i] * z_p[j] with FP64 FMAs.
+constexpr int GRAM_TP = 256;              // pixels staged per tile (one per thread)
+constexpr int GRAM_LD = GRAM_TP + 1;      // LDS row stride (floats)
+
+// Partial Gram of one GRAM_CHUNK-pixel chunk of facet q in FP64.  Per tile,
+// thread t forms z[c][t] for the 16 (padded) channels and stages it in LDS
+// as FP32 (coalesced along pixels).  Thread t then owns the 4 x 4 Gram block
+// (t % 16) / 4, (t % 16) % 4 for the pixels t / 16 + 16 n of the tile: per
+// pixel it reads 8 values and issues 16 FP64 FMAs (exact products of FP32
+// inputs), so LDS traffic is a quarter of one read pair per product.  The 16
+// pixel groups' blocks are added in a fixed order at the end.
 __global__ __launch_bounds__(GRAM_THREADS) void k_gram(FacetDev F, int L, const float* __restrict__ v0,
                                                        const float* __restrict__ xhat,
                                                        double* __restrict__ partial, int nchunk) {
-  constexpr int TP = 64;
-  __shared__ double zs[TP][NUC_L + 1];
+  __shared__ float zs[NUC_L][GRAM_LD];
+  __shared__ double red[GRAM_THREADS / 16][NUC_L * NUC_L];
   const int q = blockIdx.y, chunk = blockIdx.x;
   const int py = F.py[q], P = py * F.px[q];
   const int64_t N = (int64_t)F.Ny * F.Nx;
-  const int i = threadIdx.x / NUC_L, j = threadIdx.x % NUC_L;
-  double acc = 0.0;
+  const int blk = threadIdx.x % 16, g = threadIdx.x / 16;
+  const int i0 = 4 * (blk / 4), j0 = 4 * (blk % 4);
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
   const int p0 = chunk * GRAM_CHUNK;
-  const int p1 = min(P, p0 + GRAM_CHUNK);
-  for (int pb = p0; pb < p1; pb += TP) {
-    for (int e = threadIdx.x; e < TP * NUC_L; e += GRAM_THREADS) {
-      const int c = e / TP, pl = e % TP;
-      zs[pl][c] = (double)z_val(F, q, L, P, py, c, pb + pl < p1 ? pb + pl : P, v0, xhat, N);
+  for (int pb = p0; pb < p0 + GRAM_CHUNK; pb += GRAM_TP) {
+    {
+      const int p = pb + (int)threadIdx.x;
+      float z[NUC_L];
+#pragma unroll
+      for (int c = 0; c < NUC_L; ++c) z[c] = z_val(F, q, L, P, py, c, p < P ? p : P, v0, xhat, N);
+#pragma unroll
+      for (int c = 0; c < NUC_L; ++c) zs[c][threadIdx.x] = z[c];
     }
     __syncthreads();
-#pragma unroll 8
-    for (int pl = 0; pl < TP; ++pl) acc = fma(zs[pl][i], zs[pl][j], acc);
+#pragma unroll 4
+    for (int n = 0; n < GRAM_TP / 16; ++n) {
+      const int pl = g + 16 * n;
+      double zi[4], zj[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        zi[r] = (double)zs[i0 + r][pl];
+        zj[r] = (double)zs[j0 + r][pl];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(zi[a], zj[b], acc[a][b]);
+    }
     __syncthreads();
   }
-  partial[((int64_t)q * nchunk + chunk) * NUC_L * NUC_L + threadIdx.x] = acc;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) red[g][(i0 + a) * NUC_L + j0 + b] = acc[a][b];
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < GRAM_THREADS / 16; ++w) s += red[w][threadIdx.x];
+  partial[((int64_t)q * nchunk + chunk) * NUC_L * NUC_L + threadIdx.x] = s;
 }
 
-// FP64 sum of the per-chunk partial Grams: one workgroup per facet, thread
-// e owns Gram entry e and keeps 8 independent accumulators so the chunk
-// loads stay in flight (a single dependent chain is latency-bound).
-__global__ __launch_bounds__(NUC_L * NUC_L) void k_gram_reduce(const double* __restrict__ partial, int nchunk,
-                                                               const int* __restrict__ nchunk_q,
-                                                               double* __restrict__ gram) {
-  const int q = blockIdx.x, e = threadIdx.x;
+// FP64 sum of the per-chunk partial Grams: one workgroup of 1024 threads per
+// facet; thread t adds the chunks k = t / 256 (mod 4) of entry t % 256 with
+// 4 independent accumulators (loads stay in flight), and the four quarters
+// are added in a fixed order.
+__global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__ partial, int nchunk,
+                                                      const int* __restrict__ nchunk_q,
+                                                      double* __restrict__ gram) {
+  __shared__ double quarter[4][NUC_L * NUC_L];
+  const int q = blockIdx.x, e = threadIdx.x % (NUC_L * NUC_L), part = threadIdx.x / (NUC_L * NUC_L);
   const int nc = nchunk_q[q];
   const double* pq = partial + (int64_t)q * nchunk * NUC_L * NUC_L + e;
-  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int k = 0;
-  for (; k + 8 <= nc; k += 8) {
+  double a[4] = {0, 0, 0, 0};
+  int k = part;
+  for (; k + 12 < nc; k += 16) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a[j] += pq[(int64_t)(k + j) * NUC_L * NUC_L];
+    for (int j = 0; j < 4; ++j) a[j] += pq[(int64_t)(k + 4 * j) * NUC_L * NUC_L];
   }
-  for (; k < nc; ++k) a[0] += pq[(int64_t)k * NUC_L * NUC_L];
-  gram[(int64_t)q * NUC_L * NUC_L + e] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  for (; k < nc; k += 4) a[0] += pq[(int64_t)k * NUC_L * NUC_L];
+  quarter[part][e] = (a[0] + a[1]) + (a[2] + a[3]);
+  __syncthreads();
+  if (part == 0)
+    gram[(int64_t)q * NUC_L * NUC_L + e] = (quarter[0][e] + quarter[1][e]) + (quarter[2][e] + quarter[3][e]);
 }
 
 // One wavefront per facet.  mode 0: dual step (writes M); mode 1: singular
@@ -366,8 +409,7 @@ static void lowrank_run(fhs_ctx* ctx, fhs_lowrank* R, const float* v0, const flo
   }
   {
     Scope sc(ctx, "nuclear_eig");
-    k_gram_reduce<<<R->Q, NUC_L * NUC_L, 0, ctx->stream>>>(R->d_partial, R->nchunk, R->d_geom + 4 * R->Q,
-                                                           R->d_gram);
+    k_gram_reduce<<<R->Q, 1024, 0, ctx->stream>>>(R->d_partial, R->nchunk, R->d_geom + 4 * R->Q, R->d_gram);
     check_launch();
     k_eig<<<R->Q, 64, 0, ctx->stream>>>(R->L, R->d_gram, R->d_V, R->d_thresh, mode, R->d_M, R->d_sv);
     check_launch();
