@@ -64,7 +64,7 @@ constexpr int GRAM_LD = GRAM_TP + 1;      // LDS row stride (floats)
 
 // Partial Gram of one GRAM_CHUNK-pixel chunk of facet q in FP64.  Per tile,
 // thread t forms z[c][t] for the 16 (padded) channels and stages it in LDS
-// as FP32 (coalesced along pixels).  Thread t then owns the 4 x 4 Gram block
+// as FP32 (coalesced along pixels; the next tile is loaded meanwhile).  Thread t then owns the 4 x 4 Gram block
 // (t % 16) / 4, (t % 16) % 4 for the pixels t / 16 + 16 n of the tile: per
 // pixel it reads 8 values and issues 16 FP64 FMAs (exact products of FP32
 // inputs), so LDS traffic is a quarter of one read pair per product.  The 16
@@ -85,16 +85,19 @@ __global__ __launch_bounds__(GRAM_THREADS) void k_gram(FacetDev F, int L, const 
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
   const int p0 = chunk * GRAM_CHUNK;
+  // the next tile's values are loaded into registers while this one is summed
+  float z[NUC_L];
+  auto load = [&](int pb) {
+    const int p = pb + (int)threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < NUC_L; ++c) z[c] = z_val(F, q, L, P, py, c, p < P ? p : P, v0, xhat, N);
+  };
+  load(p0);
   for (int pb = p0; pb < p0 + GRAM_CHUNK; pb += GRAM_TP) {
-    {
-      const int p = pb + (int)threadIdx.x;
-      float z[NUC_L];
 #pragma unroll
-      for (int c = 0; c < NUC_L; ++c) z[c] = z_val(F, q, L, P, py, c, p < P ? p : P, v0, xhat, N);
-#pragma unroll
-      for (int c = 0; c < NUC_L; ++c) zs[c][threadIdx.x] = z[c];
-    }
+    for (int c = 0; c < NUC_L; ++c) zs[c][threadIdx.x] = z[c];
     __syncthreads();
+    if (pb + GRAM_TP < p0 + GRAM_CHUNK) load(pb + GRAM_TP);
 #pragma unroll 4
     for (int n = 0; n < GRAM_TP / 16; ++n) {
       const int pl = g + 16 * n;

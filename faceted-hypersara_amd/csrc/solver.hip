@@ -42,8 +42,9 @@ int lowrank_channels(const fhs_lowrank* R);
 struct fhs_pdfb {
   fhs_ctx* ctx = nullptr;
   fhs_ctx own{};            // private context state bound to the engine stream
-  fhs_ctx side{};           // second stream: the prior steps run beside the data step
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  fhs_ctx side{};           // second stream: the sparsity prior step runs beside the data step
+  fhs_ctx low{};            // third stream: the low-rank prior step
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join_low = nullptr;
   bool concurrent = true;
   fhs_data* D = nullptr;
   fhs_sara* S = nullptr;
@@ -69,24 +70,32 @@ namespace fhs {
 namespace {
 
 // After the primal update the two prior steps (reading xhat) and the data
-// step (reading x) are independent: with `concurrent` they are forked onto
-// the side stream and joined before the next iteration's primal update.
+// step (reading x) are independent: with `concurrent` the sparsity step is
+// forked onto the side stream and the low-rank step onto a third one (its
+// one-wavefront eigen-solver is latency, not work), and both are joined
+// before the next iteration's primal update.
 void run_iteration(fhs_pdfb* P) {
   fhs_ctx* c = &P->own;
   fhs_ctx* p = P->concurrent ? &P->side : c;
+  fhs_ctx* r = P->concurrent ? &P->low : c;
   const auto& b = P->b;
   primal_update(c, b.x, b.g1, b.g2, b.g0, b.xhat, P->N * P->L, P->d_scal, P->truth, P->d_scal + 4);
   if (P->concurrent) {
     FHS_HIP(hipEventRecord(P->ev_fork, c->stream));
     FHS_HIP(hipStreamWaitEvent(p->stream, P->ev_fork, 0));
+    if (P->R) FHS_HIP(hipStreamWaitEvent(r->stream, P->ev_fork, 0));
   }
-  if (P->R) lowrank_dual_step(p, P->R, b.v0, b.xhat, P->sigma00, b.g0);
+  if (P->R) lowrank_dual_step(r, P->R, b.v0, b.xhat, P->sigma00, b.g0);
   sara_dual_step(p, P->S, b.v1, b.xhat, P->L, b.weights1, P->beta1, P->sigma11, b.g1);
   data_step(c, P->D, (float2*)b.v2, (const float2*)b.y, b.x, (float2*)b.Gx_old, (float2*)b.proj, b.pU,
             P->fb_max, P->fb_min, P->fb_eps, P->dr, b.Lambda, b.g2, P->d_scal + 2);
   if (P->concurrent) {
     FHS_HIP(hipEventRecord(P->ev_join, p->stream));
     FHS_HIP(hipStreamWaitEvent(c->stream, P->ev_join, 0));
+    if (P->R) {
+      FHS_HIP(hipEventRecord(P->ev_join_low, r->stream));
+      FHS_HIP(hipStreamWaitEvent(c->stream, P->ev_join_low, 0));
+    }
   }
 }
 
@@ -131,8 +140,11 @@ int fhs_pdfb_create(fhs_ctx* ctx, fhs_data* D, fhs_sara* S, fhs_lowrank* R,
     FHS_HIP(hipStreamCreate(&P->own.stream));
     FHS_HIP(hipStreamCreate(&P->side.stream));
     P->side.device = ctx->device;
+    FHS_HIP(hipStreamCreate(&P->low.stream));
+    P->low.device = ctx->device;
     FHS_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
     FHS_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+    FHS_HIP(hipEventCreateWithFlags(&P->ev_join_low, hipEventDisableTiming));
     P->D = D;
     P->S = S;
     P->R = R;
@@ -158,14 +170,19 @@ int fhs_pdfb_destroy(fhs_pdfb* P) {
   (void)hipFree(P->d_scal);
   (void)hipHostFree(P->h_scal);
   (void)hipStreamSynchronize(P->side.stream);
+  (void)hipStreamSynchronize(P->low.stream);
   P->own.ws.release();
   P->side.ws.release();
+  P->low.ws.release();
   for (auto e : P->own.event_pool) (void)hipEventDestroy(e);
   for (auto e : P->side.event_pool) (void)hipEventDestroy(e);
+  for (auto e : P->low.event_pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(P->ev_fork);
   (void)hipEventDestroy(P->ev_join);
+  (void)hipEventDestroy(P->ev_join_low);
   (void)hipStreamDestroy(P->own.stream);
   (void)hipStreamDestroy(P->side.stream);
+  (void)hipStreamDestroy(P->low.stream);
   delete P;
   return FHS_OK;
 }
@@ -230,6 +247,7 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
     FHS_CHECK(n_iter >= 1, "pdfb_iterate: n_iter >= 1");
     c->timing = P->ctx->timing;
     P->side.timing = P->ctx->timing;
+    P->low.timing = P->ctx->timing;
     for (int it = 0; it < n_iter; ++it) {
       const bool graph_now = P->use_graph && !c->timing && P->iters >= 1;
       if (graph_now && !P->exec) {
@@ -251,6 +269,7 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
     if (c->timing) {
       merge_timing(c, P->ctx);
       merge_timing(&P->side, P->ctx);
+      merge_timing(&P->low, P->ctx);
     }
   });
   return st;
