@@ -27,7 +27,7 @@ namespace {
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
 template <int N, int B, int NT, bool INV>
-__device__ __forceinline__ void fft_lds(float2* s, const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft_lds(float2* s, const float2* tw) {
   const int tid = threadIdx.x;
   constexpr int LOG = ilog2(N);
   int Ns = 1;
@@ -103,6 +103,19 @@ __device__ __forceinline__ void fft_lds(float2* s, const float2* __restrict__ tw
   }
 }
 
+// The radix-4 stages read twiddles tw[r k tstride] < 3N/4: up to N = 2048
+// they are staged in LDS once per workgroup (read behind the data loads'
+// barrier) instead of a cache round trip in every stage; at N = 4096 the
+// table would halve the x-pass occupancy, so the stages read it from cache.
+template <int N>
+constexpr int TW_LDS = (N < 4 || N > 2048) ? 1 : 3 * N / 4;
+template <int N, int NT>
+__device__ __forceinline__ const float2* stage_twiddles(const float2* __restrict__ tw, float2* tws) {
+  if constexpr (N > 2048) return tw;
+  for (int i = threadIdx.x; i < TW_LDS<N>; i += NT) tws[i] = tw[i];
+  return tws;
+}
+
 template <int N>
 constexpr int nt_contig() { return N / 4 < 64 ? 64 : (N / 4 > 1024 ? 1024 : N / 4); }
 template <int N>
@@ -121,7 +134,9 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_fwd_y(const float* __restric
                                                          float2* __restrict__ T) {
   constexpr int NT = nt_contig<N>();
   __shared__ float2 s[N];
+  __shared__ float2 tws[TW_LDS<N>];
   const int ix = blockIdx.x, l = blockIdx.y;
+  const float2* twp = stage_twiddles<N, NT>(tw, tws);
   const float* xc = x + ((size_t)l * Nx + ix) * Ny;
   const float scx = sx[ix];
   constexpr int PER = (N + NT - 1) / NT;
@@ -137,7 +152,7 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_fwd_y(const float* __restric
     if (i < N) s[i] = make_float2(v[j], 0.f);
   }
   __syncthreads();
-  fft_lds<N, 1, NT, false>(s, tw);
+  fft_lds<N, 1, NT, false>(s, twp);
   float2* out = T + ((size_t)l * Nx + ix) * N;
   for (int i = threadIdx.x; i < N; i += NT) out[i] = s[i];
 }
@@ -150,7 +165,9 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restr
   constexpr int B = b_strided<N>();
   constexpr int NT = nt_strided<N>();
   __shared__ float2 s[N * B];
+  __shared__ float2 tws[TW_LDS<N>];
   const int ky0 = blockIdx.x * B, l = blockIdx.y;
+  const float2* twp = stage_twiddles<N, NT>(tw, tws);
   const float2* Tl = T + (size_t)l * Nx * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
   float2 v[PER];
@@ -163,7 +180,7 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restr
 #pragma unroll
   for (int j = 0; j < PER; ++j) s[threadIdx.x + j * NT] = v[j];
   __syncthreads();
-  fft_lds<N, B, NT, false>(s, tw);
+  fft_lds<N, B, NT, false>(s, twp);
   float2* F = Fx + (size_t)l * N * Kty;
   for (int t = threadIdx.x; t < N * B; t += NT) {
     const int b = t % B, k = t / B;
@@ -179,7 +196,9 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
   constexpr int B = b_strided<N>();
   constexpr int NT = nt_strided<N>();
   __shared__ float2 s[N * B];
+  __shared__ float2 tws[TW_LDS<N>];
   const int ky0 = blockIdx.x * B, l = blockIdx.y;
+  const float2* twp = stage_twiddles<N, NT>(tw, tws);
   const float2* gl = g + (size_t)l * N * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
   float2 v[PER];
@@ -191,7 +210,7 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
 #pragma unroll
   for (int j = 0; j < PER; ++j) s[threadIdx.x + j * NT] = v[j];
   __syncthreads();
-  fft_lds<N, B, NT, true>(s, tw);
+  fft_lds<N, B, NT, true>(s, twp);
   float2* Tl = T + (size_t)l * Nx * Kty;
   for (int t = threadIdx.x; t < Nx * B; t += NT) {
     const int b = t % B, i = t / B;
@@ -209,7 +228,9 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_adj_y(const float2* __restri
                                                          float* __restrict__ out) {
   constexpr int NT = nt_contig<N>();
   __shared__ float2 s[N];
+  __shared__ float2 tws[TW_LDS<N>];
   const int ix = blockIdx.x, l = blockIdx.y;
+  const float2* twp = stage_twiddles<N, NT>(tw, tws);
   const float2* Tc = T + ((size_t)l * Nx + ix) * N;
   constexpr int PER = (N + NT - 1) / NT;
   float2 v[PER];
@@ -224,7 +245,7 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_adj_y(const float2* __restri
     if (i < N) s[i] = v[j];
   }
   __syncthreads();
-  fft_lds<N, 1, NT, true>(s, tw);
+  fft_lds<N, 1, NT, true>(s, twp);
   const float sc = (scale ? scale[l] : 1.f) * sx[ix];
   float* o = out + ((size_t)l * Nx + ix) * Ny;
   for (int i = threadIdx.x; i < Ny; i += NT) o[i] = sc * sy[i] * s[i].x;
