@@ -242,7 +242,8 @@ struct SynIn {
 // loads of the next tile with the arithmetic of the current one:
 //   syn_load_t   issues the loads of the 4 subband tiles (CN x CN) into registers,
 //   syn_store_t  writes them to LDS (cs[s][cx][cy]; a barrier must follow),
-//   syn_compute_t accumulates the tile (WT*WT/WTHREADS values per thread) into acc.
+//   syn_compute_t accumulates the tile (WT*WT/WTHREADS values per thread, placed
+//                 as syn_acc_coord says) into acc.
 // Output i takes the LF/2 coefficients k = floor(i/2) + m (m < LF/2) with
 // filter tap j = (i & 1) + LF - 2 - 2m; the staged tile is zero beyond the
 // subband, so no clamping is needed.
@@ -280,7 +281,19 @@ __device__ __forceinline__ void syn_store_t(const float* cv, float* cs) {
   }
 }
 
-// expects cs staged and the caller's filter writes visible (after a barrier)
+// Output coordinates of a thread's accumulator entry q of syn_compute_t: the
+// thread makes the column pair ixl = 2c, 2c + 1 of row iyl for the pair
+// items t = threadIdx.x + (q / 2) WTHREADS (c = t / WT, iyl = t % WT).
+__device__ __forceinline__ void syn_acc_coord(int q, int& iyl, int& ixl) {
+  const int t = threadIdx.x + (q >> 1) * WTHREADS;
+  iyl = t % WT;
+  ixl = 2 * (t / WT) + (q & 1);
+}
+
+// expects cs staged and the caller's filter writes visible (after a barrier).
+// The two outputs of a parity pair (2u, 2u + 1) take the same LF/2
+// coefficients with the even and the odd taps, so each item makes both and
+// reads every staged value once for two outputs (half the LDS reads).
 template <int LF>
 __device__ __forceinline__ void syn_compute_t(const float* lo, const float* hi, int iy0, int ix0, int py, int px,
                                               const float* cs, float* Ly, float* Hy, float* acc) {
@@ -292,39 +305,46 @@ __device__ __forceinline__ void syn_compute_t(const float* lo, const float* hi, 
     rhi[j] = hi[j];
   }
   // y pass: Ly[cx][iy] = sum_k lo a[k][cx] + hi h[k][cx];  Hy: v, d
-  for (int t = threadIdx.x; t < WT * CN; t += WTHREADS) {
-    const int iyl = t % WT, cxl = t / WT;
-    const int par = (iy0 + iyl) & 1;
-    const int kb = iyl >> 1;   // floor(i/2) - cy0 (iy0 even)
-    const float* a0 = cs + (0 * CN + cxl) * CN + kb;
-    const float* h0 = cs + (1 * CN + cxl) * CN + kb;
-    const float* v0 = cs + (2 * CN + cxl) * CN + kb;
-    const float* d0 = cs + (3 * CN + cxl) * CN + kb;
-    float sl = 0.f, sh = 0.f;
+  // (iy0 is even: output 2u + par takes k = u + m with tap par + LF - 2 - 2m)
+  for (int t = threadIdx.x; t < (WT / 2) * CN; t += WTHREADS) {
+    const int u = t % (WT / 2), cxl = t / (WT / 2);
+    const float* a0 = cs + (0 * CN + cxl) * CN + u;
+    const float* h0 = cs + (1 * CN + cxl) * CN + u;
+    const float* v0 = cs + (2 * CN + cxl) * CN + u;
+    const float* d0 = cs + (3 * CN + cxl) * CN + u;
+    float sl0 = 0.f, sh0 = 0.f, sl1 = 0.f, sh1 = 0.f;
 #pragma unroll
     for (int m = 0; m < LF / 2; ++m) {
-      const int j = par + LF - 2 - 2 * m;
-      const float fa = rlo[j], fh = rhi[j];
-      sl += fa * a0[m] + fh * h0[m];
-      sh += fa * v0[m] + fh * d0[m];
+      const int je = LF - 2 - 2 * m, jo = je + 1;
+      const float a = a0[m], h = h0[m], v = v0[m], d = d0[m];
+      sl0 += rlo[je] * a + rhi[je] * h;
+      sh0 += rlo[je] * v + rhi[je] * d;
+      sl1 += rlo[jo] * a + rhi[jo] * h;
+      sh1 += rlo[jo] * v + rhi[jo] * d;
     }
-    Ly[cxl * WT + iyl] = sl;
-    Hy[cxl * WT + iyl] = sh;
+    Ly[cxl * WT + 2 * u] = sl0;
+    Hy[cxl * WT + 2 * u] = sh0;
+    Ly[cxl * WT + 2 * u + 1] = sl1;
+    Hy[cxl * WT + 2 * u + 1] = sh1;
   }
   __syncthreads();
-  // x pass
-  int q = 0;
-  for (int t = threadIdx.x; t < WT * WT; t += WTHREADS, ++q) {
-    const int iyl = t % WT, ixl = t / WT;
-    const int par = (ix0 + ixl) & 1;
-    const int kb = ixl >> 1;
-    float s = 0.f;
+  // x pass: column pair (2c, 2c + 1) of row iyl (ix0 is even)
+#pragma unroll
+  for (int q2 = 0; q2 < WT * WT / 2 / WTHREADS; ++q2) {
+    const int t = threadIdx.x + q2 * WTHREADS;
+    const int iyl = t % WT, c = t / WT;
+    float s0 = 0.f, s1 = 0.f;
 #pragma unroll
     for (int m = 0; m < LF / 2; ++m) {
-      const int j = par + LF - 2 - 2 * m;
-      s += rlo[j] * Ly[(kb + m) * WT + iyl] + rhi[j] * Hy[(kb + m) * WT + iyl];
+      const int je = LF - 2 - 2 * m, jo = je + 1;
+      const float ly = Ly[(c + m) * WT + iyl], hy = Hy[(c + m) * WT + iyl];
+      s0 += rlo[je] * ly + rhi[je] * hy;
+      s1 += rlo[jo] * ly + rhi[jo] * hy;
     }
-    if (iy0 + iyl < py && ix0 + ixl < px) acc[q] += s;
+    if (iy0 + iyl < py) {
+      if (ix0 + 2 * c < px) acc[2 * q2] += s0;
+      if (ix0 + 2 * c + 1 < px) acc[2 * q2 + 1] += s1;
+    }
   }
   __syncthreads();
 }
@@ -452,9 +472,11 @@ __global__ __launch_bounds__(WTHREADS) void k_syn_lk(SaraGeom G, int k, const fl
   for (int q = 0; q < WT * WT / WTHREADS; ++q) acc[q] = 0.f;
   syn_tile(in, lf, flo, fhi, iy0, ix0, py, px, cs, Ly, Hy, acc);
   float* o = sc + G.scr[b][k - 1];
-  int q = 0;
-  for (int t = threadIdx.x; t < WT * WT; t += WTHREADS, ++q) {
-    const int iy = iy0 + t % WT, ix = ix0 + t / WT;
+#pragma unroll
+  for (int q = 0; q < WT * WT / WTHREADS; ++q) {
+    int iyl, ixl;
+    syn_acc_coord(q, iyl, ixl);
+    const int iy = iy0 + iyl, ix = ix0 + ixl;
     if (iy < py && ix < px) o[iy + (int64_t)py * ix] = acc[q];
   }
 }
@@ -548,9 +570,11 @@ __global__ __launch_bounds__(WTHREADS) void k_syn_l1_std(SaraGeom G, const float
   syn_l1_std_bases<0>(G, v1l, sc, iy0, ix0, flo, fhi, cs, Ly, Hy, cv, acc);
   const float* dir = v1l + G.boff[G.nb];
   float* o = out + (int64_t)l * G.Ny * G.Nx;
-  int q = 0;
-  for (int t = threadIdx.x; t < WT * WT; t += WTHREADS, ++q) {
-    const int iy = iy0 + t % WT, ix = ix0 + t / WT;
+#pragma unroll
+  for (int q = 0; q < WT * WT / WTHREADS; ++q) {
+    int iyl, ixl;
+    syn_acc_coord(q, iyl, ixl);
+    const int iy = iy0 + iyl, ix = ix0 + ixl;
     if (iy < G.Ny && ix < G.Nx) {
       const int64_t p = iy + (int64_t)G.Ny * ix;
       float s = acc[q] * G.norm;
