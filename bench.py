@@ -47,6 +47,8 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-solve', action='store_true', help='skip the time-to-SNR solve')
     ap.add_argument('--solve-seconds', type=float, default=60.0)
+    ap.add_argument('--solve-repeats', type=int, default=3,
+                    help='timed solves per start after the full reference solve; the median time is reported')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
@@ -185,23 +187,38 @@ def main():
     if not args.no_solve:
         from fhs import warmstart
         prm = dict(param, track_snr=True)
+        # the full plain solve fixes the reference SNR (and is the untimed
+        # warm-up); the times come from --solve-repeats further solves of the
+        # same deterministic sequence, each stopped once it reaches ref - 0.1 dB
         _, plain = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds)
         ref = plain['snr'][-1]
         tols = (0.1, 0.5, 1.0, 2.0, 3.0)
 
-        def reach(info, extra_s=0.0):
-            out = {}
-            for d in tols:
-                k = warmstart.iterations_to(info['snr'], ref - d)
-                out[f'ref-{d}dB'] = None if k is None else {'iterations': k,
-                                                            's': round(info['time'][k - 1] + extra_s, 4)}
-            return out
+        def timed(**kw):
+            def stop(t, eng, info):
+                return not (info['snr'] and info['snr'][-1] >= ref - min(tols))
+            return [solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds,
+                                   callback=stop, **kw)[1] for _ in range(max(1, args.solve_repeats))]
 
+        def reach(runs, extra_s=0.0):
+            out, every = {}, {}
+            for d in tols:
+                k = warmstart.iterations_to(runs[0]['snr'], ref - d)
+                if k is None or any(len(r['time']) < k for r in runs):
+                    out[f'ref-{d}dB'] = None
+                    continue
+                ts = [r['time'][k - 1] + extra_s for r in runs]
+                out[f'ref-{d}dB'] = {'iterations': k, 's': round(float(np.median(ts)), 4)}
+                every[f'ref-{d}dB'] = [round(t, 4) for t in ts]
+            return out, every
+
+        p_reach, p_all = reach(timed())
         solve = {'ref_snr_db': round(ref, 3), 'ref_definition': 'SNR of the plain run at convergence '
                  '(3 reweighting steps, reference stopping rules); SNR = 20 log10(||X0||/||X0 - X||)',
+                 'timing': f'median of {max(1, args.solve_repeats)} solves after the full reference solve',
                  'plain': {'iterations': plain['iterations'], 'wall_s': round(plain['wall'], 3),
                            'converged': plain['converged'], 'reweights': plain['reweights'],
-                           'reach': reach(plain)}}
+                           'reach': p_reach, 'reach_s_each': p_all}}
         wpath = warmstart.default_weights_path()
         if os.path.exists(wpath):
             ws = warmstart.WarmStart.load(ctx, wpath)
@@ -214,13 +231,13 @@ def main():
             X0d = solvers._cube(prob.X0).double()
             x0_snr = 20 * np.log10(torch.linalg.vector_norm(X0d).item() /
                                    torch.linalg.vector_norm(X0d - x0.double()).item())
-            target = ref - 0.1
             _, warm = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, x_init=x0, weights_from_init=False,
                                      time_limit=args.solve_seconds)
+            w_reach, w_all = reach(timed(x_init=x0, weights_from_init=False), t_pred)
             solve['warm_start'] = {'network': os.path.relpath(wpath, ROOT), 'predict_s': round(t_pred, 4),
                                    'x_init_snr_db': round(x0_snr, 3),
                                    'iterations': warm['iterations'], 'final_snr_db': round(warm['snr'][-1], 3),
-                                   'reach': reach(warm, t_pred)}
+                                   'reach': w_reach, 'reach_s_each': w_all}
             # the chosen form per tolerance: whichever start reaches it first
             solve['chosen'] = {}
             for d in tols:
