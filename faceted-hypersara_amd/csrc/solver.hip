@@ -57,7 +57,14 @@ struct fhs_pdfb {
   int fb_max = 20, fb_min = 1, dr = 0;
   double fb_eps = 1e-8;
   double* d_scal = nullptr;   // [5]: the four stopping-rule sums, then sum (truth - x)^2
-  double* h_scal = nullptr;   // pinned [5]
+  struct Pub {                // coherent pinned host memory, written by k_publish
+    double scal[5];
+    unsigned long long seq;
+  };
+  Pub* h_pub = nullptr;
+  Pub* d_pub = nullptr;       // its device address
+  unsigned long long* d_seq = nullptr;   // iterations published (device counter)
+  unsigned long long seen = 0;           // last sequence number read by the host
   const float* truth = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -68,6 +75,22 @@ struct fhs_pdfb {
 
 namespace fhs {
 namespace {
+
+// Last node of an iteration: copies the five sums into coherent host memory
+// and then advances the published sequence number (system-scope release),
+// so the host learns that the iteration ended by polling host memory instead
+// of a stream synchronisation and a separate copy.
+__global__ void k_publish(const double* __restrict__ d_scal, unsigned long long* __restrict__ d_seq,
+                          fhs_pdfb::Pub* pub) {
+  if (threadIdx.x == 0) {
+    const unsigned long long s = *d_seq + 1;
+    *d_seq = s;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pub->scal[k] = d_scal[k];
+    __threadfence_system();
+    __hip_atomic_store(&pub->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
 
 // After the primal update the two prior steps (reading xhat) and the data
 // step (reading x) are independent: with `concurrent` the sparsity step is
@@ -95,6 +118,25 @@ void run_iteration(fhs_pdfb* P) {
     if (P->R) {
       FHS_HIP(hipEventRecord(P->ev_join_low, r->stream));
       FHS_HIP(hipStreamWaitEvent(c->stream, P->ev_join_low, 0));
+    }
+  }
+  k_publish<<<1, 64, 0, c->stream>>>(P->d_scal, P->d_seq, P->d_pub);
+  check_launch();
+}
+
+// Waits until iteration `want` is published.  Polls the host word; every
+// 1024 polls the stream is queried, so a failed launch raises its error and
+// an idle stream without the publication is reported instead of waited for.
+void wait_published(fhs_pdfb* P, unsigned long long want) {
+  for (unsigned long spin = 1;; ++spin) {
+    if (__atomic_load_n(&P->h_pub->seq, __ATOMIC_ACQUIRE) >= want) return;
+    if ((spin & 1023) == 0) {
+      const hipError_t q = hipStreamQuery(P->own.stream);
+      if (q == hipSuccess) {
+        if (__atomic_load_n(&P->h_pub->seq, __ATOMIC_ACQUIRE) >= want) return;
+        throw Error(FHS_ERR_HIP, "pdfb_iterate: the stream is idle but the iteration was not published");
+      }
+      if (q != hipErrorNotReady) FHS_HIP(q);
     }
   }
 }
@@ -156,8 +198,12 @@ int fhs_pdfb_create(fhs_ctx* ctx, fhs_data* D, fhs_sara* S, fhs_lowrank* R,
     if (R) FHS_CHECK(lowrank_channels(R) == P->L, "pdfb_create: low-rank prior has a different L");
     FHS_HIP(hipMalloc(&P->d_scal, 5 * sizeof(double)));
     FHS_HIP(hipMemset(P->d_scal, 0, 5 * sizeof(double)));
-    FHS_HIP(hipHostMalloc(&P->h_scal, 5 * sizeof(double), hipHostMallocDefault));
-    for (int k = 0; k < 5; ++k) P->h_scal[k] = 0.0;
+    FHS_HIP(hipMalloc(&P->d_seq, sizeof(unsigned long long)));
+    FHS_HIP(hipMemset(P->d_seq, 0, sizeof(unsigned long long)));
+    FHS_HIP(hipHostMalloc((void**)&P->h_pub, sizeof(fhs_pdfb::Pub), hipHostMallocMapped | hipHostMallocCoherent));
+    for (int k = 0; k < 5; ++k) P->h_pub->scal[k] = 0.0;
+    P->h_pub->seq = 0;
+    FHS_HIP(hipHostGetDevicePointer((void**)&P->d_pub, P->h_pub, 0));
     *out = P;
   });
 }
@@ -168,7 +214,8 @@ int fhs_pdfb_destroy(fhs_pdfb* P) {
   (void)hipStreamSynchronize(P->own.stream);
   drop_graph(P);
   (void)hipFree(P->d_scal);
-  (void)hipHostFree(P->h_scal);
+  (void)hipFree(P->d_seq);
+  (void)hipHostFree(P->h_pub);
   (void)hipStreamSynchronize(P->side.stream);
   (void)hipStreamSynchronize(P->low.stream);
   P->own.ws.release();
@@ -260,10 +307,9 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
         FHS_HIP(hipGraphLaunch(P->exec, c->stream));
       else
         run_iteration(P);
-      FHS_HIP(hipMemcpyAsync(P->h_scal, P->d_scal, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      FHS_HIP(hipStreamSynchronize(c->stream));
+      wait_published(P, ++P->seen);
       if (scalars)
-        for (int k = 0; k < 4; ++k) scalars[4 * it + k] = P->h_scal[k];
+        for (int k = 0; k < 4; ++k) scalars[4 * it + k] = P->h_pub->scal[k];
       ++P->iters;
     }
     if (c->timing) {
@@ -289,7 +335,7 @@ int fhs_pdfb_truth_error(fhs_pdfb* P, double* err2) {
   return guard(P->ctx, [&] {
     FHS_CHECK(P->truth, "pdfb_truth_error: no reference cube set (fhs_pdfb_set_truth)");
     FHS_CHECK(P->iters > 0, "pdfb_truth_error: no iteration has run");
-    *err2 = P->h_scal[4];
+    *err2 = P->h_pub->scal[4];
   });
 }
 
