@@ -64,7 +64,7 @@ def test_direct_and_adjoint(ctx, Ny, Nx, M, layout):
 
 
 @pytest.mark.parametrize('Ny,Nx', [(8, 8), (16, 32), (64, 64), (100, 60), (256, 256), (512, 512), (1024, 512),
-                                   (512, 1024), (256, 2048)])
+                                   (512, 1024), (256, 2048), (64, 2048), (2048, 64)])
 def test_scaled_fourier(ctx, Ny, Nx):
     import fhs
     rng = np.random.default_rng(Ny + Nx)
