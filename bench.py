@@ -195,10 +195,8 @@ def main():
         tols = (0.1, 0.5, 1.0, 2.0, 3.0)
 
         def timed(**kw):
-            def stop(t, eng, info):
-                return not (info['snr'] and info['snr'][-1] >= ref - min(tols))
             return [solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds,
-                                   callback=stop, **kw)[1] for _ in range(max(1, args.solve_repeats))]
+                                   stop_snr=ref - min(tols), **kw)[1] for _ in range(max(1, args.solve_repeats))]
 
         def reach(runs, extra_s=0.0):
             out, every = {}, {}

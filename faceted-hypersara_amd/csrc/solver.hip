@@ -16,6 +16,8 @@
 // a hipGraph and replayed; parameters that change (thresholds after a
 // reweighting, epsilon) live in device memory, so replays need no re-capture.
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -286,39 +288,97 @@ int fhs_pdfb_set_concurrent(fhs_pdfb* P, int enable) {
   return FHS_OK;
 }
 
+}  // extern "C"
+
+namespace fhs {
+namespace {
+
+// One iteration: graph replay (captured on first use) or eager launches, then
+// the wait for its published scalars.
+void step(fhs_pdfb* P) {
+  fhs_ctx* c = &P->own;
+  const bool graph_now = P->use_graph && !c->timing && P->iters >= 1;
+  if (graph_now && !P->exec) {
+    FHS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    run_iteration(P);
+    FHS_HIP(hipStreamEndCapture(c->stream, &P->graph));
+    FHS_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+  }
+  if (graph_now)
+    FHS_HIP(hipGraphLaunch(P->exec, c->stream));
+  else
+    run_iteration(P);
+  wait_published(P, ++P->seen);
+  ++P->iters;
+}
+
+void begin_calls(fhs_pdfb* P) {
+  P->own.timing = P->ctx->timing;
+  P->side.timing = P->ctx->timing;
+  P->low.timing = P->ctx->timing;
+}
+
+void end_calls(fhs_pdfb* P) {
+  if (P->own.timing) {
+    merge_timing(&P->own, P->ctx);
+    merge_timing(&P->side, P->ctx);
+    merge_timing(&P->low, P->ctx);
+  }
+}
+
+}  // namespace
+}  // namespace fhs
+
+extern "C" {
+
 int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
   if (!P) return FHS_ERR_ARG;
-  fhs_ctx* c = &P->own;
-  int st = guard(P->ctx, [&] {
+  return guard(P->ctx, [&] {
     FHS_CHECK(P->params_set, "pdfb_iterate: call fhs_pdfb_set_params first");
     FHS_CHECK(n_iter >= 1, "pdfb_iterate: n_iter >= 1");
-    c->timing = P->ctx->timing;
-    P->side.timing = P->ctx->timing;
-    P->low.timing = P->ctx->timing;
+    begin_calls(P);
     for (int it = 0; it < n_iter; ++it) {
-      const bool graph_now = P->use_graph && !c->timing && P->iters >= 1;
-      if (graph_now && !P->exec) {
-        FHS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        run_iteration(P);
-        FHS_HIP(hipStreamEndCapture(c->stream, &P->graph));
-        FHS_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
-      }
-      if (graph_now)
-        FHS_HIP(hipGraphLaunch(P->exec, c->stream));
-      else
-        run_iteration(P);
-      wait_published(P, ++P->seen);
+      step(P);
       if (scalars)
         for (int k = 0; k < 4; ++k) scalars[4 * it + k] = P->h_pub->scal[k];
-      ++P->iters;
     }
-    if (c->timing) {
-      merge_timing(c, P->ctx);
-      merge_timing(&P->side, P->ctx);
-      merge_timing(&P->low, P->ctx);
-    }
+    end_calls(P);
   });
-  return st;
+}
+
+int fhs_pdfb_run(fhs_pdfb* P, int rule, int k0, int min_iter, int max_iter, double rel_var, double fid_tol,
+                 double rel_var_low, int n_max, double err2_stop, double seconds_max, double* out, int* n_done,
+                 int* converged) {
+  if (!P || !out || !n_done || !converged) return FHS_ERR_ARG;
+  return guard(P->ctx, [&] {
+    FHS_CHECK(P->params_set, "pdfb_run: call fhs_pdfb_set_params first");
+    FHS_CHECK(n_max >= 1 && (rule == 0 || rule == 1), "pdfb_run: bad argument");
+    FHS_CHECK(err2_stop < 0.0 || P->truth, "pdfb_run: err2_stop needs fhs_pdfb_set_truth");
+    begin_calls(P);
+    const auto t0 = std::chrono::steady_clock::now();
+    *n_done = 0;
+    *converged = 0;
+    for (int i = 0; i < n_max; ++i) {
+      step(P);
+      const double* sc = P->h_pub->scal;
+      const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      for (int j = 0; j < 5; ++j) out[6 * i + j] = sc[j];
+      out[6 * i + 5] = secs;
+      *n_done = i + 1;
+      // the PDFB stopping rule of the solver (NaN compares false, as in MATLAB)
+      const double rel = sc[1] > 0.0 ? std::sqrt(sc[0]) / std::sqrt(sc[1]) : std::nan("");
+      const double nres = std::sqrt(sc[2]), neps = std::sqrt(sc[3]);
+      const int k = k0 + i + 1;
+      const bool small = rule == 1 ? (rel <= rel_var && nres <= fid_tol * neps) : rel <= rel_var;
+      if (k >= min_iter && (k >= max_iter || small || rel <= rel_var_low)) {
+        *converged = 1;
+        break;
+      }
+      if (err2_stop >= 0.0 && sc[4] <= err2_stop) break;
+      if (seconds_max > 0.0 && secs > seconds_max) break;
+    }
+    end_calls(P);
+  });
 }
 
 int fhs_pdfb_set_truth(fhs_pdfb* P, const float* X0) {

@@ -89,6 +89,8 @@ PROTOTYPES = {
     'fhs_pdfb_iterate': [c_void_p, c_int, c_double_p],
     'fhs_pdfb_norm_res': [c_void_p, c_double_p],
     'fhs_pdfb_set_truth': [c_void_p, c_void_p],
+    'fhs_pdfb_run': [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double, c_double, c_int, c_double,
+                     c_double, c_double_p, c_int_p, c_int_p],
     'fhs_pdfb_truth_error': [c_void_p, c_double_p],
     'fhs_pdfb_stream': [c_void_p, C.POINTER(c_void_p)],
     'fhs_data_forward': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -239,6 +239,19 @@ class PdfbEngine:
                        'pdfb_iterate')
         return out.reshape(n, 4)
 
+    def run(self, rule, k0, prm, n_max, err2_stop=-1.0, seconds_max=0.0):
+        """Iterations until the PDFB stopping rule (``fhs_pdfb_run``): returns
+        (scalars (n, 6): four sums, ||X0 - x||^2, seconds since the call;
+        converged)."""
+        out = np.zeros(6 * max(1, int(n_max)))
+        n, conv = C.c_int(), C.c_int()
+        self.ctx.check(lib().fhs_pdfb_run(
+            self.h, int(rule), int(k0), int(prm['pdfb_min_iter']), int(prm['pdfb_max_iter']),
+            float(prm['pdfb_rel_var']), float(prm['pdfb_fidelity_tolerance']), float(prm['pdfb_rel_var_low']),
+            int(n_max), float(err2_stop), float(seconds_max), out.ctypes.data_as(C.POINTER(C.c_double)),
+            C.byref(n), C.byref(conv)), 'pdfb_run')
+        return out[:6 * n.value].reshape(n.value, 6), bool(conv.value)
+
     def set_truth(self, X0):
         """Reference cube (L, Nx, Ny) float32 on the GPU, or None: the primal
         update then also sums ||X0 - x||^2 (see ``truth_error``)."""
@@ -286,13 +299,18 @@ def snr_db(X0, x):
 
 
 def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, graph=True,
-           max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True):
+           max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True, stop_snr=None):
     """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'.
 
     With a nonzero ``x_init`` the reference derives the first weights from it
     (``weights_from_init=True``, ``src/fhs/fhs_initialize_dual_and_weights.m``);
     ``weights_from_init=False`` keeps the zero start's unit weights, so only
-    the starting point of the same sequence of problems changes."""
+    the starting point of the same sequence of problems changes.
+
+    Without a ``callback`` the iterations of a reweighting cycle run inside
+    the library (``fhs_pdfb_run``: the same stopping rule, no host round trip
+    per iteration); ``stop_snr`` (with X0) ends the solve at the first
+    iteration whose SNR reaches it."""
     prm = dict(DEFAULT_PARAM)
     prm.update(param or {})
     ctx = meas.ctx
@@ -367,6 +385,52 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         nX0 = float(torch.linalg.vector_norm(eng._truth.double()))
     t0 = time.perf_counter()
     t = 0
+
+    def end_of_cycle(t):
+        """Reweighting at the end of a PDFB cycle; True when the solve is done."""
+        nonlocal x_last, rw_count, last_step
+        nl = torch.linalg.vector_norm(x_last.double()).item()
+        rel_rw = (torch.linalg.vector_norm((x_last - eng.x).double()).item() / nl) if nl > 0 else float('inf')
+        x_last = eng.x.clone()
+        if (rw_count >= prm['reweighting_min_iter'] and
+                (rw_count >= prm['reweighting_max_iter'] or rel_rw <= prm['reweighting_rel_var'])):
+            info['converged'] = True
+            return True
+        _reweight(eng, alpha, sig, sig_bar)
+        if lowrank is not None:
+            eng.set_params(beta0[:, None] * eng._w0, beta1, sigma00, sigma11, sigma22, meas.epsilon,
+                           prm['elipse_proj_max_iter'], prm['elipse_proj_min_iter'], prm['elipse_proj_eps'])
+        rw_count += 1
+        last_step = t
+        info['reweights'].append(t)
+        return False
+
+    rule = 1 if kind in ('fhs', 'sara') else 0
+    if callback is None:
+        # whole cycles inside the library
+        err2_stop = -1.0
+        if track and stop_snr is not None:
+            err2_stop = (nX0 / 10 ** (stop_snr / 20)) ** 2
+        while t < max_iter:
+            tc = time.perf_counter() - t0
+            left = 0.0 if time_limit is None else max(time_limit - tc, 1e-9)
+            out, conv = eng.run(rule, t - last_step, prm, max_iter - t, err2_stop, left)
+            for sc in out:
+                info['rel_val'].append(sqrt(sc[0]) / sqrt(sc[1]) if sc[1] > 0 else float('nan'))
+                info['norm_res'].append(sqrt(sc[2]))
+                info['norm_eps'].append(sqrt(sc[3]))
+                info['time'].append(tc + sc[5])
+                if track:
+                    info['snr'].append(20 * np.log10(nX0 / sqrt(sc[4])) if sc[4] > 0 else float('inf'))
+            t += len(out)
+            if conv:
+                if end_of_cycle(t):
+                    break
+                continue
+            break   # n_max, the SNR target or the time limit
+        info['iterations'] = t
+        info['wall'] = time.perf_counter() - t0
+        return eng, info
     for t in range(1, max_iter + 1):
         s = eng.iterate(1)[0]
         rel_val = sqrt(s[0]) / sqrt(s[1]) if s[1] > 0 else float('nan')
@@ -390,23 +454,14 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         else:
             conv = (k >= prm['pdfb_min_iter']) and (
                 k >= prm['pdfb_max_iter'] or rel_val <= prm['pdfb_rel_var'] or rel_val <= prm['pdfb_rel_var_low'])
+        if conv:
+            if end_of_cycle(t):
+                break
+            continue
+        if track and stop_snr is not None and info['snr'][-1] >= stop_snr:
+            break
         if time_limit is not None and info['time'][-1] > time_limit:
             break
-        if conv:
-            nl = torch.linalg.vector_norm(x_last.double()).item()
-            rel_rw = (torch.linalg.vector_norm((x_last - eng.x).double()).item() / nl) if nl > 0 else float('inf')
-            x_last = eng.x.clone()
-            if (rw_count >= prm['reweighting_min_iter'] and
-                    (rw_count >= prm['reweighting_max_iter'] or rel_rw <= prm['reweighting_rel_var'])):
-                info['converged'] = True
-                break
-            _reweight(eng, alpha, sig, sig_bar)
-            if lowrank is not None:
-                eng.set_params(beta0[:, None] * eng._w0, beta1, sigma00, sigma11, sigma22, meas.epsilon,
-                               prm['elipse_proj_max_iter'], prm['elipse_proj_min_iter'], prm['elipse_proj_eps'])
-            rw_count += 1
-            last_step = t
-            info['reweights'].append(t)
     info['iterations'] = t
     info['wall'] = time.perf_counter() - t0
     return eng, info

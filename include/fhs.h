@@ -314,6 +314,22 @@ int fhs_pdfb_iterate(fhs_pdfb* p, int n_iter, double* scalars);
 int fhs_pdfb_set_truth(fhs_pdfb* p, const float* X0);
 /* last iteration's sum (X0 - x)^2 (host copy, no GPU call) */
 int fhs_pdfb_truth_error(fhs_pdfb* p, double* err2);
+/* Runs iterations until the PDFB stopping rule of the solver holds for
+ * iteration k = k0 + i (src/fhs/facetHyperSARA.m:877-881, rule 1, also
+ * src/sara/sara.m; src/hs/hyperSARA.m, rule 0):
+ *   k >= min_iter && (k >= max_iter || small || rel_x <= rel_var_low),
+ *   small = rel_x <= rel_var (&& norm_res <= fid_tol * norm_eps for rule 1),
+ * rel_x = sqrt(sum dx^2 / sum x^2); or until n_max iterations, or until
+ * sum (X0 - x)^2 <= err2_stop (err2_stop >= 0; needs fhs_pdfb_set_truth),
+ * or after seconds_max seconds of wall-clock (> 0).  The per-iteration loop
+ * of the solvers without a host round trip per iteration.
+ * out: host double[6 * n_max]: per iteration the four sums of
+ * fhs_pdfb_iterate, sum (X0 - x)^2 (0 without a reference cube) and the
+ * seconds since the call began; n_done: iterations run; converged: 1 if the
+ * stopping rule held at the last one. */
+int fhs_pdfb_run(fhs_pdfb* p, int rule, int k0, int min_iter, int max_iter, double rel_var, double fid_tol,
+                 double rel_var_low, int n_max, double err2_stop, double seconds_max, double* out,
+                 int* n_done, int* converged);
 /* last iteration's per-block residual norms ||G_b A x - y_b||: host double[nblocks] */
 int fhs_pdfb_norm_res(fhs_pdfb* p, double* norm_res);
 int fhs_pdfb_stream(fhs_pdfb* p, void** stream);

@@ -132,3 +132,26 @@ def test_fused_snr_tracking(ctx):
     assert np.allclose(info['snr'], host, rtol=0, atol=1e-4), (info['snr'], host)
     x_u, _ = solvers.hyper_sara(meas, {'squared_operator_norm': son, 'nu2': nu2}, max_total_iter=15)
     assert np.array_equal(x_t, x_u)
+
+
+def test_library_cycles_match_per_iteration_loop(ctx):
+    """Cycles run inside the library (fhs_pdfb_run) take the same decisions
+    and iterates as the per-iteration loop (forced by a callback), including
+    the reweighting steps, and stop_snr ends both at the same iteration."""
+    from fhs import solvers, synth
+    prob = synth.make_problem(64, 64, 4, 4000, 2, seed=23, n_sources=30)
+    meas = solvers.Measurement(ctx, prob, precond=True)
+    son, nu2 = meas.op_norm(False), meas.op_norm(True)
+    p = {'squared_operator_norm': son, 'nu2': nu2, 'track_snr': True, 'pdfb_max_iter': 40,
+         'reweighting_max_iter': 2}
+    x_a, ia = solvers.hyper_sara(meas, p, X0=prob.X0)
+    x_b, ib = solvers.hyper_sara(meas, p, X0=prob.X0, callback=lambda t, eng, info: None)
+    assert ia['iterations'] == ib['iterations'] and ia['reweights'] == ib['reweights']
+    assert ia['reweights'], 'the case should reweight'
+    assert np.array_equal(x_a, x_b)
+    assert np.array_equal(np.array(ia['rel_val']), np.array(ib['rel_val']), equal_nan=True)
+    assert np.allclose(ia['snr'], ib['snr'], rtol=0, atol=1e-9)
+    target = ia['snr'][len(ia['snr']) // 2]
+    _, sa = solvers._solve('hs', meas, p, X0=prob.X0, stop_snr=target)
+    _, sb = solvers._solve('hs', meas, p, X0=prob.X0, stop_snr=target, callback=lambda t, eng, info: None)
+    assert sa['iterations'] == sb['iterations'] and sa['snr'][-1] >= target
