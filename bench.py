@@ -53,14 +53,18 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--config', choices=('hs512', 'fhs1024'), default='hs512',
+    ap.add_argument('--config', choices=('hs512', 'fhs1024', 'sara256'), default='hs512',
                     help='hs512: HyperSARA 512x512x16 (the headline workload); fhs1024: one 16-channel spectral '
                          'sub-cube of Faceted HyperSARA 1024x1024x32 (C = 2), 4x4 nuclear-norm facets with 25%% '
-                         'overlap, 300k-500k visibilities per channel')
+                         'overlap, 300k-500k visibilities per channel; sara256: SARA 256x256, 65,536 '
+                         'visibilities, 2 blocks, 300 iterations, against the FP64 oracle run to the same end '
+                         '(the CPU-runnable case: its SNR is the reference)')
     a = ap.parse_args()
     if a.config == 'fhs1024':
         a.size, a.channels, a.vis = 1024, 16, 500000
         a.no_cpu = True     # one FP64 oracle iteration of this size takes minutes
+    if a.config == 'sara256':
+        a.size, a.channels, a.vis = 256, 1, 65536
     return a
 
 
@@ -100,6 +104,54 @@ def algorithmic_bytes(meas, sara_s):
     }
 
 
+def sara_against_oracle(args, solvers, meas, prob, son, nu2, n_iter=300):
+    """SARA 256x256: the plain FP32 GPU run and the FP64 numpy oracle, both
+    n_iter iterations without reweighting from the same data.  The oracle's
+    final SNR is the reference; returns (solve, cpu_baseline)."""
+    from threadpoolctl import threadpool_limits
+    from fhs import warmstart
+    from oracle.solvers import OracleProblem, sara_solver
+    prm = {'squared_operator_norm': son, 'nu2': nu2, 'pdfb_min_iter': n_iter, 'pdfb_max_iter': n_iter,
+           'reweighting_min_iter': 0, 'reweighting_max_iter': 0, 'track_snr': True}
+    _, full = solvers._solve('sara', meas, prm, X0=prob.X0)           # untimed: reference curve, warm-up
+    b = np.concatenate([[0], np.cumsum(prob.blocks[0])])
+    blocks = [[(int(b[i]), int(b[i + 1])) for i in range(len(prob.blocks[0]))]]
+    with threadpool_limits(limits=1):
+        op = OracleProblem(prob.Ny, prob.Nx, meas.u, meas.v, meas.y_host,
+                           [np.asarray(p, dtype=np.float64) for p in meas.pU_host], blocks, prob.epsilon,
+                           g_scale=[1.0 / s for s in prob.sigma_noise])
+        pr = dict(solvers.DEFAULT_PARAM)
+        pr.update({'nu2': nu2, 'gamma': float(np.sqrt(1.0 / son[0])), 'pdfb_max_iter': n_iter})
+        tc = time.perf_counter()
+        _, oinfo = sara_solver(op, pr, X0=prob.X0[:, :, 0], max_iter_total=n_iter)
+        tcpu = time.perf_counter() - tc
+    ref = oinfo['snr'][-1]
+    tols = (0.1, 0.5, 1.0, 2.0, 3.0)
+    runs = [solvers._solve('sara', meas, prm, X0=prob.X0, stop_snr=ref - min(tols))[1]
+            for _ in range(max(1, args.solve_repeats))]
+    gpu, cpu_reach = {}, {}
+    for d in tols:
+        key = f'ref-{d}dB'
+        k = warmstart.iterations_to(full['snr'], ref - d)
+        ok = k is not None and all(len(r['time']) >= k for r in runs)
+        gpu[key] = {'iterations': k, 's': round(float(np.median([r['time'][k - 1] for r in runs])), 4)} if ok else None
+        kc = warmstart.iterations_to(oinfo['snr'], ref - d)
+        cpu_reach[key] = None if kc is None else {'iterations': kc, 's': round(tcpu * kc / n_iter, 2)}
+    solve = {'ref_snr_db': round(ref, 3),
+             'ref_definition': f'SNR of the FP64 numpy oracle after {n_iter} SARA iterations (no reweighting) on '
+                               'the same data; SNR = 20 log10(||X0||/||X0 - X||)',
+             'gpu_fp32_snr_db_at_end': round(full['snr'][-1], 3),
+             'snr_gap_db': round(full['snr'][-1] - ref, 4),
+             'timing': f'GPU: median of {max(1, args.solve_repeats)} solves after a full one; CPU: one run, '
+                       'time at iteration k taken as k/n of the run',
+             'plain': {'iterations': full['iterations'], 'reach': gpu},
+             'cpu_oracle': {'iterations': n_iter, 'wall_s': round(tcpu, 2), 'reach': cpu_reach}}
+    cpu = {'value': round(n_iter / tcpu, 4), 'unit': 'iters/s', 'cores': 1, 'kind': 'port',
+           'sample': f'the whole {n_iter}-iteration SARA 256x256 run of the FP64 numpy oracle, single thread '
+                     f'({tcpu:.1f} s)'}
+    return solve, cpu
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,9 +180,11 @@ def main():
         prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed,
                                   flag_range=(0.0, 0.4))
         facets = facet_geometry(args.size, args.size, 4, 4, (0.25, 0.25))
+    elif args.config == 'sara256':
+        prob = synth.make_problem(args.size, args.size, 1, args.vis, 2, seed=args.seed)
     else:
         prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
-    kind = 'fhs' if facets is not None else 'hs'
+    kind = 'fhs' if facets is not None else ('sara' if args.config == 'sara256' else 'hs')
     meas = solvers.Measurement(ctx, prob, precond=True)
     son = meas.op_norm(False)
     nu2 = meas.op_norm(True)
@@ -184,7 +238,10 @@ def main():
 
     # ---- time to reference SNR: plain FP32 form (zero start) and warm start --
     solve = None
-    if not args.no_solve:
+    cpu = None
+    if args.config == 'sara256' and not args.no_solve:
+        solve, cpu = sara_against_oracle(args, solvers, meas, prob, son, nu2)
+    elif not args.no_solve:
         from fhs import warmstart
         prm = dict(param, track_snr=True)
         # the full plain solve fixes the reference SNR (and is the untimed
@@ -249,8 +306,7 @@ def main():
                     solve['chosen'][key] = {'form': 'plain', **p}
 
     # ---- CPU baseline: FP64 oracle, bounded sample -------------------------
-    cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.config != 'sara256':
         from threadpoolctl import threadpool_limits
         from oracle.solvers import OracleProblem, hypersara as hs_ref
         blocks = []
@@ -286,7 +342,9 @@ def main():
         'vs_baseline': None,
         'dtype': 'f32 (f64 norms, f64 Gram/eig)',
         'data': 'synthetic',
-        'config': {'workload': (f'HyperSARA {args.size}x{args.size}x{args.channels}, '
+        'config': {'workload': (f'SARA {args.size}x{args.size}, {args.vis} visibilities, 2 blocks'
+                                if args.config == 'sara256' else
+                                f'HyperSARA {args.size}x{args.size}x{args.channels}, '
                                 f'{args.vis} base vis/channel, 10-30% flags, 4 blocks/channel')
                    if facets is None else
                    (f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 2 spectral sub-cubes '
