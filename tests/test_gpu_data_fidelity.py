@@ -119,3 +119,69 @@ def test_split_projection_is_deterministic(ctx):
         outs.append((v2d.cpu().numpy(), projd.cpu().numpy(), np.asarray(nrd)))
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('L,nblk', [(2, 1), (1, 3)])
+def test_update_dual_data_fidelity_dr(ctx, L, nblk):
+    """The dimensionality-reduction branch (src/update_dual_data_fidelity.m:
+    78-97): holographic H = G + G' stored as its lower triangle (square, over
+    the whole grid), weights Lambda, closed-form l2-ball dual step; against
+    the FP64 oracle over consecutive iterations."""
+    import scipy.sparse as sp
+    import fhs
+    Ny = Nx = 16
+    rng = np.random.default_rng(L * 10 + nblk)
+    op = NufftOracle(np.zeros(1), np.zeros(1), Ny, Nx)
+    No = op.No
+    plan = fhs.NufftPlan(ctx, Ny, Nx, op.Kty, op.Ktx, op.sy, op.sx)
+    Gs, Gh, ys, lams, blocks = [], [], [], [], []
+    for l in range(L):
+        G = sp.random(No, No, density=8.0 / No, random_state=int(rng.integers(1 << 30)), format='coo')
+        G = sp.tril(G, format='csr').astype(np.complex128)
+        G.data = (rng.normal(size=G.nnz) + 1j * rng.normal(size=G.nnz)) * 0.1
+        G.data = G.data.astype(np.complex64).astype(np.complex128)
+        Gs.append(G)
+        Gh.append(fhs.GMatrix.from_scipy(ctx, G))
+        x0 = rng.random((Ny, Nx))
+        fx = op.A(x0)
+        ys.append(f32((G @ fx + G.conj().T @ fx) + 0.05 * (rng.normal(size=No) + 1j * rng.normal(size=No))))
+        lams.append(f32(rng.uniform(0.5, 1.5, No)))
+        b = np.linspace(0, No, nblk + 1).astype(int)
+        blocks.append([(b[i], b[i + 1]) for i in range(nblk)])
+    D = fhs.DataOperator(ctx, plan, Gh, [[b - a for a, b in bl] for bl in blocks])
+    G_c = [[Gs[l][a:b] for a, b in blocks[l]] for l in range(L)]
+    # row block of a square matrix: the oracle's test of the lower triangle
+    # needs the block's own columns; use whole-grid columns (W = all) with
+    # the Hermitian part written out explicitly instead
+    for l in range(L):
+        for j, (a, b) in enumerate(blocks[l]):
+            H = (Gs[l] + Gs[l].conj().T).tocsr()[a:b]
+            G_c[l][j] = H
+    W = [[np.arange(No)] * nblk for _ in range(L)]
+    y = [[ys[l][a:b] for a, b in blocks[l]] for l in range(L)]
+    lam = [[lams[l][a:b] for a, b in blocks[l]] for l in range(L)]
+    eps = [[(0.5 if (l + j) % 2 else 3.0) * np.linalg.norm(y[l][j]) for j in range(nblk)] for l in range(L)]
+    v2 = [[np.zeros(b - a, complex) for a, b in bl] for bl in blocks]
+    proj = [[np.zeros(b - a, complex) for a, b in bl] for bl in blocks]
+    x = np.zeros((Ny, Nx, L))
+    Fx_old = np.zeros((No, L), complex)
+    sigma22 = np.array([0.3 + 0.1 * l for l in range(L)], dtype=np.float32)
+    stack = lambda cells: torch.from_numpy(np.concatenate([np.concatenate(c) for c in cells]).astype(np.complex64)).cuda()  # noqa
+    yd, v2d, projd = stack(y), D.zeros(), D.zeros()
+    lamd = torch.from_numpy(np.concatenate([np.concatenate(c) for c in lam]).astype(np.float32)).cuda()
+    xd = torch.zeros((L, Nx, Ny), dtype=torch.float32, device='cuda')
+    gx_old = D.init_fx_old(xd, lamd, True)
+    for it in range(3):
+        x = np.maximum(x + rng.normal(size=x.shape) * 0.5, 0).astype(np.float32).astype(np.float64)
+        xd = torch.from_numpy(np.ascontiguousarray(x.transpose(2, 1, 0)).astype(np.float32)).cuda()
+        v2, Ftx, Fx_old, proj, nr, gnr, neps = dual_ref(
+            v2, y, x, Fx_old, proj, [op.A] * L, [op.At] * L, G_c, W, None, eps, 20, 1, 1e-8,
+            sigma22.astype(np.float64), flag_dimensionality_reduction=True, Lambda=lam)
+        v2d, Ftxd, gx_old, projd, nrd, gnrd, nepsd = fhs.update_dual_data_fidelity(
+            v2d, yd, xd, gx_old, projd, plan, plan, D, None, None, np.concatenate(eps), 20, 1, 1e-8, sigma22,
+            flag_dimensionality_reduction=True, Lambda=lamd)
+        ynorm = np.linalg.norm(np.concatenate([np.concatenate(c) for c in y]))
+        assert rel(v2d.cpu().numpy(), np.concatenate([np.concatenate(c) for c in v2]), ynorm) < TOL, it
+        assert rel(Ftxd.cpu().numpy().transpose(2, 1, 0), Ftx) < TOL, it
+        assert rel(nrd, np.concatenate(nr)) < TOL, it
+        assert abs(gnrd - gnr) / gnr < TOL and abs(nepsd - neps) / neps < 1e-12
