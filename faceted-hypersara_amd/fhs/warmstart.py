@@ -251,6 +251,7 @@ def main(argv=None):
     ap.add_argument('--first-seed', type=int, default=1000)
     ap.add_argument('--size', type=int, default=256)
     ap.add_argument('--vis', type=int, default=65536)
+    ap.add_argument('--channels', type=int, default=8, help='channels per training cube (the network sees groups of 8)')
     ap.add_argument('--epochs', type=int, default=150)
     ap.add_argument('--batch', type=int, default=4)
     ap.add_argument('--lr', type=float, default=1e-3)
@@ -258,8 +259,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     ctx = default_context()
     t0 = time.perf_counter()
-    pairs, recs = make_pairs(ctx, range(a.first_seed, a.first_seed + a.cubes), Ny=a.size, Nx=a.size, L=8,
-                             M_base=a.vis, param={'reweighting_max_iter': 3})
+    pairs, recs = make_pairs(ctx, range(a.first_seed, a.first_seed + a.cubes), Ny=a.size, Nx=a.size, L=a.channels,
+                             M_base=a.vis, param={'reweighting_max_iter': 3},
+                             log=lambda m: print(m, flush=True))
     print(f'{len(pairs)} pairs in {time.perf_counter() - t0:.1f} s '
           f'(mean {np.mean([r["iterations"] for r in recs]):.0f} PDFB iterations per cube)', flush=True)
     ws = WarmStart(ctx)
