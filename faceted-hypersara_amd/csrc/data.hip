@@ -95,7 +95,9 @@ __global__ __launch_bounds__(256) void k_data_forward(const GDev* __restrict__ g
 }
 
 // Holographic (DR) forward: Lambda (G + G') applied to (2 Fx - Fx_old), and
-// the residual operator on Fx, G square M x M.
+// the residual operator on Fx, G square M x M (STATE); or y = Lambda (G + G') Fx
+// into gx_old (the DR measurement operator, r2 unused).
+template <bool STATE>
 __global__ __launch_bounds__(256) void k_data_forward_dr(const GDev* __restrict__ gd,
                                                          const int64_t* __restrict__ row_off,
                                                          const float2* __restrict__ Fx, int64_t No,
@@ -112,8 +114,10 @@ __global__ __launch_bounds__(256) void k_data_forward_dr(const GDev* __restrict_
   const int64_t i = row_off[l] + m;
   const float lm = lam ? lam[i] : 1.f;
   gx = cscale(gx, lm);
-  const float2 old = gx_old[i];
-  r2[i] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
+  if (STATE) {
+    const float2 old = gx_old[i];
+    r2[i] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
+  }
   gx_old[i] = gx;
 }
 
@@ -653,7 +657,7 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
     const int units = (int)cdiv(D->maxM, 256);
     const unsigned grid = xcd_grid(L, units);
     if (dr)
-      k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2, L,
+      k_data_forward_dr<true><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2, L,
                                                        units);
     else
       launch_data_forward(ctx, D, gx_old, grid, units);
@@ -697,7 +701,7 @@ void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int
   const int units = (int)cdiv(D->maxM, 256);
   const unsigned grid = xcd_grid(D->L, units);
   if (dr)
-    k_data_forward_dr<<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2,
+    k_data_forward_dr<true><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2,
                                                      D->L, units);
   else
     launch_data_forward(ctx, D, gx_old, grid, units);
@@ -720,12 +724,22 @@ void data_forward(fhs_ctx* ctx, fhs_data* D, const float* x, const float* lam, f
   check_launch();
 }
 
-// out_l = scale_l real(At(G_l' (lam .* v)))
+// y = Lambda .* (G + G') A x (all channels, holographic G)
+void data_forward_dr(fhs_ctx* ctx, fhs_data* D, const float* x, const float* lam, float2* y) {
+  nufft_forward(ctx, D->A, x, D->L, D->Fx);
+  Scope sc(ctx, "data_forward");
+  const int units = (int)cdiv(D->maxM, 256);
+  k_data_forward_dr<false><<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No,
+                                                                          lam, y, nullptr, D->L, units);
+  check_launch();
+}
+
+// out_l = scale_l real(At(G_l' (lam .* v)))  (+ G_l (lam .* v) for the DR form)
 void data_adjoint(fhs_ctx* ctx, fhs_data* D, const float2* v, const float* lam, const float* scale_dev,
-                  float* out) {
+                  float* out, int dr = 0) {
   {
     Scope sc(ctx, "data_adjoint");
-    launch_data_adjoint(ctx, D, v, lam, 0);
+    launch_data_adjoint(ctx, D, v, lam, dr);
   }
   nufft_adjoint(ctx, D->A, D->g2, D->L, scale_dev, out);
 }
@@ -847,6 +861,30 @@ int fhs_data_adjoint(fhs_ctx* ctx, fhs_data* D, const void* y, const float* Lamb
       FHS_HIP(hipMemcpyAsync(sd, scale, D->L * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     }
     data_adjoint(ctx, D, (const float2*)y, Lambda, sd, out);
+  });
+}
+
+int fhs_data_forward_dr(fhs_ctx* ctx, fhs_data* D, const float* x, const float* Lambda, void* y) {
+  if (!ctx) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(D && x && y, "data_forward_dr: null argument");
+    FHS_CHECK(D->square, "data_forward_dr: needs square (holographic) G");
+    data_forward_dr(ctx, D, x, Lambda, (float2*)y);
+  });
+}
+
+int fhs_data_adjoint_dr(fhs_ctx* ctx, fhs_data* D, const void* y, const float* Lambda, const float* scale,
+                        float* out) {
+  if (!ctx) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(D && y && out, "data_adjoint_dr: null argument");
+    FHS_CHECK(D->square, "data_adjoint_dr: needs square (holographic) G");
+    float* sd = nullptr;
+    if (scale) {
+      sd = (float*)ctx->ws.get("data_adj_scale", D->L * sizeof(float));
+      FHS_HIP(hipMemcpyAsync(sd, scale, D->L * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    }
+    data_adjoint(ctx, D, (const float2*)y, Lambda, sd, out, 1);
   });
 }
 

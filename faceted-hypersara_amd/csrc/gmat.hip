@@ -7,6 +7,51 @@
 
 namespace fhs {
 
+// column ranges of ~ADJ_ENTRIES nonzeros (<= ADJ_COLS columns)
+std::vector<int> adjoint_units(const std::vector<int>& colptr) {
+  const int64_t ncols = (int64_t)colptr.size() - 1;
+  std::vector<int> u{0};
+  int64_t cnt = 0, cols = 0;
+  for (int64_t c = 0; c < ncols; ++c) {
+    const int64_t len = colptr[c + 1] - colptr[c];
+    if (cols > 0 && (cnt + len > ADJ_ENTRIES || cols >= ADJ_COLS)) {
+      u.push_back((int)c);
+      cnt = 0;
+      cols = 0;
+    }
+    cnt += len;
+    ++cols;
+  }
+  u.push_back((int)ncols);
+  return u;
+}
+
+// CSC of a separable G from its anchors: colptr over the Kty*Ktx grid and per
+// nonzero (row << 6) | (jx*Ky + jy), rows ascending inside every column.
+void sep_csc(int64_t M, const int32_t* anchor, int Kty, int Ktx, int Ky, int Kx, std::vector<int>& colptr,
+             std::vector<unsigned>& pk) {
+  const int64_t ncols = (int64_t)Kty * Ktx;
+  colptr.assign(ncols + 1, 0);
+  for (int64_t m = 0; m < M; ++m) {
+    const int a = anchor[m];
+    const int ky0 = a % Kty, kx0 = a / Kty;
+    for (int jx = 0; jx < Kx; ++jx)
+      for (int jy = 0; jy < Ky; ++jy) colptr[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx) + 1]++;
+  }
+  for (int64_t k = 0; k < ncols; ++k) colptr[k + 1] += colptr[k];
+  std::vector<int> pos(colptr.begin(), colptr.end() - 1);
+  pk.resize((size_t)M * Ky * Kx);
+  for (int64_t m = 0; m < M; ++m) {
+    const int a = anchor[m];
+    const int ky0 = a % Kty, kx0 = a / Kty;
+    for (int jx = 0; jx < Kx; ++jx)
+      for (int jy = 0; jy < Ky; ++jy) {
+        const int col = (ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx);
+        pk[pos[col]++] = ((unsigned)m << 6) | (unsigned)(jx * Ky + jy);
+      }
+  }
+}
+
 namespace {
 
 template <class T>
@@ -34,25 +79,6 @@ void build_csc(int64_t ncols, const std::vector<int>& rows, const std::vector<in
     crow[p] = rows[e];
     cval[p] = vals[e];
   }
-}
-
-// column ranges of ~ADJ_ENTRIES nonzeros (<= ADJ_COLS columns)
-std::vector<int> adjoint_units(const std::vector<int>& colptr) {
-  const int64_t ncols = (int64_t)colptr.size() - 1;
-  std::vector<int> u{0};
-  int64_t cnt = 0, cols = 0;
-  for (int64_t c = 0; c < ncols; ++c) {
-    const int64_t len = colptr[c + 1] - colptr[c];
-    if (cols > 0 && (cnt + len > ADJ_ENTRIES || cols >= ADJ_COLS)) {
-      u.push_back((int)c);
-      cnt = 0;
-      cols = 0;
-    }
-    cnt += len;
-    ++cols;
-  }
-  u.push_back((int)ncols);
-  return u;
 }
 
 struct TileLists {
@@ -355,25 +381,7 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
     if (tiled) {
       tl = tile_lists(M, anchor, Kty, Ktx, Ky, Kx);
     } else {
-      colptr.assign(ncols + 1, 0);
-      for (int64_t m = 0; m < M; ++m) {
-        const int a = anchor[m];
-        const int ky0 = a % Kty, kx0 = a / Kty;
-        for (int jx = 0; jx < Kx; ++jx)
-          for (int jy = 0; jy < Ky; ++jy) colptr[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx) + 1]++;
-      }
-      for (int64_t k = 0; k < ncols; ++k) colptr[k + 1] += colptr[k];
-      std::vector<int> pos(colptr.begin(), colptr.end() - 1);
-      pk.resize(nnz);
-      for (int64_t m = 0; m < M; ++m) {   // rows ascending inside every column
-        const int a = anchor[m];
-        const int ky0 = a % Kty, kx0 = a / Kty;
-        for (int jx = 0; jx < Kx; ++jx)
-          for (int jy = 0; jy < Ky; ++jy) {
-            const int col = (ky0 + jy) % Kty + Kty * ((kx0 + jx) % Ktx);
-            pk[pos[col]++] = ((unsigned)m << 6) | (unsigned)(jx * Ky + jy);
-          }
-      }
+      sep_csc(M, anchor, Kty, Ktx, Ky, Kx, colptr, pk);
       units = adjoint_units(colptr);
     }
     FHS_HIP(hipSetDevice(ctx->device));

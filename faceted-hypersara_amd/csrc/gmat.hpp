@@ -474,11 +474,17 @@ __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const fl
   }
 }
 
+// host construction helpers (gmat.hip)
+std::vector<int> adjoint_units(const std::vector<int>& colptr);
+void sep_csc(int64_t M, const int32_t* anchor, int Kty, int Ktx, int Ky, int Kx, std::vector<int>& colptr,
+             std::vector<unsigned>& pk);
+
 }  // namespace fhs
 
 struct fhs_gmat {
   int device = 0;
   fhs::GDev d{};
   int64_t M = 0, ncols = 0, nnz = 0;
+  int64_t n_modes = 0;   // holographic form: active Fourier modes (0 for a measurement G)
   void* mem[14] = {};
 };

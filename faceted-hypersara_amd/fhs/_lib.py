@@ -47,6 +47,9 @@ PROTOTYPES = {
     'fhs_gmat_info': [c_void_p, c_i64_p, c_i64_p, c_i64_p],
     'fhs_apply_direct_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_apply_adjoint_operator': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    'fhs_gmat_create_holographic': [c_void_p, c_void_p, C.POINTER(c_void_p)],
+    'fhs_gmat_holographic_info': [c_void_p, c_void_p, c_i64_p, c_void_p],
+    'fhs_holographic_grid': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_nufft_create_host': [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p,
                               C.POINTER(c_void_p)],
     'fhs_nufft_interp_coefficients': [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p, c_int, c_int,
@@ -95,6 +98,8 @@ PROTOTYPES = {
     'fhs_pdfb_stream': [c_void_p, C.POINTER(c_void_p)],
     'fhs_data_forward': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_data_adjoint': [c_void_p, c_void_p, c_void_p, c_void_p, c_float_p, c_void_p],
+    'fhs_data_forward_dr': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    'fhs_data_adjoint_dr': [c_void_p, c_void_p, c_void_p, c_void_p, c_float_p, c_void_p],
     'fhs_sara_update_weights': [c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p],
     'fhs_nn_gemm': [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int,
                     C.c_float, C.c_float, c_void_p, c_int],

@@ -195,7 +195,9 @@ class PdfbEngine:
         self.v1 = torch.zeros((L, sara.s), **f32)
         self.v2 = meas.D.zeros()
         self.proj = meas.D.zeros()
-        self.gx_old = meas.D.init_fx_old(self.x)
+        self.dr = bool(getattr(meas, 'dr', False))
+        self.Lambda = getattr(meas, 'Lambda', None) if self.dr else None
+        self.gx_old = meas.D.init_fx_old(self.x, self.Lambda, self.dr)
         self.weights1 = torch.ones(sara.s, **f32)
         b = _Buffers()
         b.Ny, b.Nx = Ny, Nx
@@ -205,7 +207,7 @@ class PdfbEngine:
         b.y = _vp(meas.y)
         b.pU = _vp(meas.pU)
         b.weights1 = _vp(self.weights1)
-        b.Lambda = None
+        b.Lambda = _vp(self.Lambda)
         self._b = b
         h = C.c_void_p()
         ctx.check(lib().fhs_pdfb_create(ctx.h, meas.D.h, sara.h, lowrank.h if lowrank else None,
@@ -213,7 +215,7 @@ class PdfbEngine:
         self.h = h
         torch.cuda.synchronize()
 
-    def set_params(self, thresh0, beta1, sigma00, sigma11, sigma22, epsilon, fb_max, fb_min, fb_eps, dr=False):
+    def set_params(self, thresh0, beta1, sigma00, sigma11, sigma22, epsilon, fb_max, fb_min, fb_eps, dr=None):
         L = self.meas.L
         th = None
         if self.lowrank is not None:
@@ -222,6 +224,7 @@ class PdfbEngine:
             th = th_arr.ctypes.data_as(C.POINTER(C.c_double))
         s22 = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma22, dtype=np.float32), (L,)))
         eps = np.ascontiguousarray(np.asarray(epsilon, dtype=np.float64))
+        dr = self.dr if dr is None else dr
         self.ctx.check(lib().fhs_pdfb_set_params(
             self.h, th, float(beta1), float(sigma00), float(sigma11), s22.ctypes.data_as(C.POINTER(C.c_float)),
             eps.ctypes.data_as(C.POINTER(C.c_double)), int(fb_max), int(fb_min), float(fb_eps), int(dr)),
@@ -339,7 +342,9 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         nu2 = meas.op_norm(meas.pU is not None)
     son = np.asarray(son, dtype=np.float64)
     nu2 = np.asarray(nu2, dtype=np.float64)
-    std = np.ones(L)
+    # whitened data: unit noise std (the DR form measures it on its gridded noise)
+    std = getattr(meas, 'noise_std', None)
+    std = np.ones(L) if std is None else np.asarray(std, dtype=np.float64)
     alpha = prm['reweighting_alpha']
     if kind == 'sara':
         sig = sqrt(std[0] ** 2 / son[0])

@@ -102,6 +102,24 @@ int fhs_apply_direct_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* Fx,
 int fhs_apply_adjoint_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* y,
                                const float* Lambda, void* Fx);
 
+/* ------------------------------------- holographic (DR) data operator -- */
+/* Dimensionality-reduced form of one channel's separable G, built on the
+ * GPU (lib/operators/util_load_data_gen_dr_measurement_operator.m:196-225):
+ * H = tril(G'G) restricted to the active Fourier modes W = {p : (G'G)_pp >
+ * 0} (:203) with its diagonal halved (:215-222).  H is square over the
+ * whole Kty*Ktx grid with the rows and columns off W empty (W ascending, so
+ * tril over W's order is tril over grid order); data vectors of the DR form
+ * are grid-sized and zero off W, their entries on W being the reference's
+ * compact vectors.  Deterministic (no atomics). */
+int fhs_gmat_create_holographic(fhs_ctx* ctx, const fhs_gmat* G, fhs_gmat** H);
+/* n_modes = |W|; Lambda (device float[Kty*Ktx], may be NULL) <- 1 ./
+ * sqrt(diag(G'G)) on W (:206), 0 off W. */
+int fhs_gmat_holographic_info(fhs_ctx* ctx, const fhs_gmat* H, int64_t* n_modes, float* Lambda);
+/* yh = Lambda .* (G' y) on the grid (device complex[M] -> complex[Kty*Ktx]):
+ * the gridded data (:171, :209), or with y a noise vector the gridded noise
+ * whose norm is the DR l2 bound (:190-192, :211-212). */
+int fhs_holographic_grid(fhs_ctx* ctx, const fhs_gmat* G, const fhs_gmat* H, const void* y, void* yh);
+
 /* ------------------------------------------------- scaled Fourier (A) -- */
 /* Plan for A = fft2(pad(sn .* x)) with sn = sy * sx' (real scaling factors
  * of the NUFFT, host arrays sy[Ny], sx[Nx]); grid Kty x Ktx, powers of two
@@ -182,6 +200,13 @@ int fhs_data_forward(fhs_ctx* ctx, fhs_data* data, const float* x, const float* 
  * (scale: host float[L] or NULL). */
 int fhs_data_adjoint(fhs_ctx* ctx, fhs_data* data, const void* y, const float* Lambda,
                      const float* scale, float* out);
+/* The same pair with flag_visibility_gridding on a lower-triangular
+ * (holographic) G: y = Lambda .* (G + G') A x and out_l = scale_l
+ * real(At((G' + G)(Lambda .* y))) (lib/operators/HS_forward_operator_G.m,
+ * HS_adjoint_operator_G.m, the istril branch).  Square G only. */
+int fhs_data_forward_dr(fhs_ctx* ctx, fhs_data* data, const float* x, const float* Lambda, void* y);
+int fhs_data_adjoint_dr(fhs_ctx* ctx, fhs_data* data, const void* y, const float* Lambda,
+                        const float* scale, float* out);
 /* [v2, Ftx, Fx_old, proj, norm_res, global_norm_res, norm_epsilon] =
  *   update_dual_data_fidelity(v2, y, x, Fx_old, proj, A, At, G, W, pU,
  *     epsilon, elipse_proj_max_iter, elipse_proj_min_iter, elipse_proj_eps,

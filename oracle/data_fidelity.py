@@ -131,3 +131,44 @@ def compute_residual_images(x, y, A, At, G, W, flag_visibility_gridding=False, L
                 rd[Wij] += Gij.conj().T @ (Gij @ Fd[Wij])
         out[:, :, i] = np.real(At[i](r)) / np.max(np.real(At[i](rd)))
     return out
+
+
+def holographic_operator(G, y=None, noise=None):
+    """Dimensionality-reduced data of one channel,
+    ``lib/operators/util_load_data_gen_dr_measurement_operator.m:160-225``
+    (one data set; the reference sums the same terms over its data sets).
+
+    ``G``: scipy sparse (M visibilities x No grid points), the measurement
+    rows with W = the whole grid (the reference's ``G_curr(W_curr, :) =
+    G{1}'`` places them so, ``:167-168``).  Flagged rows are assumed removed
+    (the reference masks them with ``abs(nW) > 0``, ``:171``).
+
+    Returns ``(H, W, Lambda, y_grid, l2bound, sigma)``:
+      H       scipy csr |W| x |W|: ``tril(G_curr * G_curr')`` restricted to
+              ``W`` with its diagonal halved (``:196, :215-222``);
+      W       ascending grid indices with ``diag > 0`` (``:200-203``);
+      Lambda  ``1 ./ sqrt(diag(W))`` (``:206``);
+      y_grid  ``(G_curr * y)(W) .* Lambda`` (``:171, :209``), None without y;
+      l2bound ``norm((G_curr * noise)(W) .* Lambda)`` and sigma its
+              ``std`` (``:186-192, :211-213``), None without noise.
+
+    Pinned (``tests/test_oracle.py``) by a dense ``G^H G`` brute force on
+    small random matrices: ``H + H^H`` equals it on ``W x W``, H is lower
+    triangular, ``Lambda^2 diag = 1``, W are the non-empty columns of G.
+    """
+    import scipy.sparse as sp
+    Gc = sp.csr_matrix(G).conj().T.tocsr()          # G_curr: No x M
+    Hfull = sp.tril(Gc @ Gc.conj().T).tocsr()
+    diagH = np.real(Hfull.diagonal())
+    W = np.flatnonzero(diagH > 0)
+    Lambda = 1.0 / np.sqrt(diagH[W])
+    H = Hfull[W][:, W].tocsr()
+    H.setdiag(0.5 * H.diagonal())
+    H.eliminate_zeros()
+    y_grid = None if y is None else (Gc @ np.asarray(y))[W] * Lambda
+    l2bound = sigma = None
+    if noise is not None:
+        gn = (Gc @ np.asarray(noise))[W] * Lambda
+        l2bound = float(np.linalg.norm(gn))
+        sigma = float(np.std(gn, ddof=1))
+    return H, W, Lambda, y_grid, l2bound, sigma

@@ -38,6 +38,28 @@ class OracleProblem:
         self.pU = [[np.asarray(pU[l][a:b]) for (a, b) in blocks[l]] for l in range(self.L)]
         self.epsilon = [list(e) for e in epsilon]
         self.blocks = blocks
+        self.dr, self.Lambda = False, None
+
+    def holographic(self, noise):
+        """The DR form (``imaging/imaging.m:619-631``): per channel one block
+        holding H, W, Lambda, the gridded data and the l2 bound of the
+        gridded ``noise[l]`` (``data_fidelity.holographic_operator``); no
+        preconditioning.  Returns a new problem."""
+        from .data_fidelity import holographic_operator
+        import copy
+        P = copy.copy(self)
+        P.G, P.W, P.y, P.pU, P.epsilon, P.Lambda = [], [], [], [], [], []
+        for l, op in enumerate(self.ops):
+            y = np.concatenate(self.y[l])
+            H, W, lam, yg, l2, _ = holographic_operator(op.G, y, noise[l])
+            P.G.append([H])
+            P.W.append([W])
+            P.y.append([yg])
+            P.pU.append([np.ones(len(W))])
+            P.epsilon.append([l2])
+            P.Lambda.append([lam])
+        P.dr = True
+        return P
 
 
 def hs_initial_weights(x, sara, alpha, sig, sig_bar):
@@ -115,7 +137,7 @@ def hypersara(problem, param, x_init=None, X0=None, max_iter_total=None, callbac
         v2, g2, Fx_old, proj, norm_res, gnr, neps = update_dual_data_fidelity(
             v2, P.y, xsol, Fx_old, proj, P.A, P.At, P.G, P.W, P.pU, P.epsilon,
             param['elipse_proj_max_iter'], param['elipse_proj_min_iter'],
-            param['elipse_proj_eps'], sigma22)
+            param['elipse_proj_eps'], sigma22, flag_dimensionality_reduction=P.dr, Lambda=P.Lambda)
         g = g0 + g1 + g2
         nx = np.linalg.norm(xsol.ravel())
         rel_val = np.linalg.norm((prev - xsol).ravel()) / nx if nx > 0 else np.nan

@@ -270,6 +270,45 @@ def test_data_fidelity_step_dense_composition():
     assert np.allclose(r2, 2 * Gd @ Fx - Gd @ Fx_old[:, 0])
 
 
+@pytest.mark.parametrize('kind', ['random', 'nufft'])
+def test_holographic_operator_dense(kind):
+    """The DR data (H = tril(G'G)(W, W) with a halved diagonal, W, Lambda,
+    gridded data and noise bound) against a dense G^H G brute force."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(21)
+    if kind == 'random':
+        M, No = 40, 30
+        G = sp.random(M, No, density=0.15, random_state=3, format='csr').astype(np.complex128)
+        G.data = rng.normal(size=G.nnz) + 1j * rng.normal(size=G.nnz)
+        G = G.tolil()
+        G[:, 7] = 0                                   # an empty column: not in W
+        G = G.tocsr()
+        G.eliminate_zeros()
+    else:
+        op = onu.NufftOracle(rng.normal(0, 0.8, 50), rng.normal(0, 0.8, 50), 8, 8)
+        G, No, M = op.G.tocsr(), op.No, 50
+    y = rng.normal(size=M) + 1j * rng.normal(size=M)
+    n = (rng.normal(size=M) + 1j * rng.normal(size=M)) / np.sqrt(2)
+    H, W, lam, yg, l2, sig = odf.holographic_operator(G, y, n)
+    Gd = G.toarray()
+    GhG = Gd.conj().T @ Gd
+    Wd = np.flatnonzero(np.linalg.norm(Gd, axis=0) > 0)
+    assert np.array_equal(W, Wd)
+    if kind == 'random':
+        assert 7 not in W and len(W) < No
+    Hd = H.toarray()
+    assert np.allclose(np.triu(Hd, 1), 0)
+    assert np.allclose(Hd + Hd.conj().T, GhG[np.ix_(W, W)])
+    assert np.allclose(lam ** 2 * np.real(np.diag(GhG))[W], 1.0)
+    assert np.allclose(yg, lam * (Gd.conj().T @ y)[W])
+    assert np.isclose(l2, np.linalg.norm(lam * (Gd.conj().T @ n)[W]))
+    gn = lam * (Gd.conj().T @ n)[W]
+    assert np.isclose(sig, np.sqrt(np.sum(np.abs(gn - gn.mean()) ** 2) / (len(gn) - 1)))
+    # the DR operator Lambda (H + H') Lambda is Lambda G'G Lambda on W
+    f = rng.normal(size=len(W)) + 1j * rng.normal(size=len(W))
+    assert np.allclose(lam * (H @ f + H.conj().T @ f), lam * (GhG[np.ix_(W, W)] @ f))
+
+
 def test_dirty_image_of_point_source_is_normalised_psf():
     rng = np.random.default_rng(9)
     Ny = Nx = 16
