@@ -48,6 +48,9 @@ struct fhs_data {
   int fb_split = 1;                    // workgroups per block in the FB projection
   unsigned* fb_sync = nullptr;         // device [2 nblocks + 1]: arrival counts, generations, error flag
   double* fb_part = nullptr;           // device [nblocks][2][FB_SPLIT_MAX][FB_PART], then [nblocks][FB_SPLIT_MAX]
+  bool stencil = false;                // DR form applied through the holographic stencils
+  int l2_split = 1;                    // workgroups per block in the DR dual step
+  double* l2_part = nullptr;           // device [nblocks][L2_SPLIT_MAX][2] (square G only)
 };
 
 namespace fhs {
@@ -119,6 +122,130 @@ __global__ __launch_bounds__(256) void k_data_forward_dr(const GDev* __restrict_
     r2[i] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
   }
   gx_old[i] = gx;
+}
+
+// Holographic form applied as a stencil (G square over the grid, with its
+// full Hermitian stencil G'G = H + H'): per grid point p of channel l,
+//   s_p = sum over the neighbourhood offsets o of stencil[o][act[p]] in(p + o)
+// (grid wrap).  DR_STATE: in = Fx, gx = Lambda_p s_p, r2 = 2 gx - gx_old,
+// gx_old <- gx (rows off W are left alone: they stay zero); DR_APPLY:
+// y_p = Lambda_p s_p, 0 off W; DR_ADJ: in = Lambda .* v2, g2_p = s_p, 0 off W.
+enum DrMode { DR_STATE = 0, DR_APPLY = 1, DR_ADJ = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dr_stencil(const GDev* __restrict__ gd,
+                                                    const int64_t* __restrict__ row_off,
+                                                    const float2* __restrict__ in, int64_t No,
+                                                    const float* __restrict__ lam, float2* __restrict__ out,
+                                                    float2* __restrict__ r2, int L, int units) {
+  int l, u;
+  if (!xcd_map(L, units, l, u)) return;
+  const GDev g = gd[l];
+  const int p = u * blockDim.x + threadIdx.x;
+  if (p >= g.M) return;
+  const int64_t base = row_off[l];
+  const int i = g.act[p];
+  if (i < 0) {
+    if (MODE != DR_STATE) out[(MODE == DR_ADJ ? (int64_t)l * No : base) + p] = make_float2(0.f, 0.f);
+    return;
+  }
+  const float2* __restrict__ src = MODE == DR_ADJ ? in + base : in + (size_t)l * No;
+  const float* __restrict__ lm = lam ? lam + base : nullptr;
+  const int py = p % g.Kty, px = p / g.Kty;
+  const int ny = 2 * g.Ky - 1, nx = 2 * g.Kx - 1;
+  const float2* __restrict__ st = g.stencil + i;
+  float ax = 0.f, ay = 0.f;
+  for (int ox = 0; ox < nx; ++ox) {
+    int qx = px + ox - (g.Kx - 1);
+    qx += qx < 0 ? g.Ktx : (qx >= g.Ktx ? -g.Ktx : 0);
+    const int col = qx * g.Kty;
+    for (int oy = 0; oy < ny; ++oy) {
+      int qy = py + oy - (g.Ky - 1);
+      qy += qy < 0 ? g.Kty : (qy >= g.Kty ? -g.Kty : 0);
+      const float2 h = st[(size_t)(oy + ny * ox) * g.nact];
+      float2 v = src[col + qy];
+      if (MODE == DR_ADJ && lm) v = cscale(v, lm[col + qy]);
+      ax += h.x * v.x - h.y * v.y;
+      ay += h.x * v.y + h.y * v.x;
+    }
+  }
+  float2 s = make_float2(ax, ay);
+  if (MODE == DR_ADJ) {
+    out[(int64_t)l * No + p] = s;
+    return;
+  }
+  if (lm) s = cscale(s, lm[p]);
+  if (MODE == DR_APPLY) {
+    out[base + p] = s;
+  } else {
+    const float2 old = out[base + p];
+    r2[base + p] = make_float2(2.f * s.x - old.x, 2.f * s.y - old.y);
+    out[base + p] = s;
+  }
+}
+
+// DR dual step v2 <- sc(v2 + r2 - y, eps) (closed-form l2 prox, sc(z, r) =
+// z (1 - min(r/||z||, 1))) and the residual ||gx - y||, over many
+// workgroups: each block's rows are cut into S equal
+// chunks; pass 1 sums ||v2 + r2 - y||^2 and ||gx - y||^2 per chunk, pass 2
+// adds a block's chunk sums in chunk order (every workgroup of the block the
+// same way) and scales its own chunk.
+constexpr int L2_SPLIT_MAX = 256;
+constexpr int L2_THREADS = 256;
+
+__global__ __launch_bounds__(L2_THREADS) void k_l2_partial(const int64_t* __restrict__ blk_start, int S,
+                                                           const float2* __restrict__ v2,
+                                                           const float2* __restrict__ y,
+                                                           const float2* __restrict__ r2,
+                                                           const float2* __restrict__ gx,
+                                                           double* __restrict__ part) {
+  __shared__ double sh[2 * L2_THREADS / 64];
+  const int b = blockIdx.x / S, c = blockIdx.x % S;
+  const int64_t a0 = blk_start[b], a1 = blk_start[b + 1];
+  const int64_t chunk = (a1 - a0 + S - 1) / S;
+  const int64_t lo = a0 + c * chunk, hi = lo + chunk < a1 ? lo + chunk : a1;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += L2_THREADS) {
+    const float2 vv = v2[i], rr = r2[i], yy = y[i], gg = gx[i];
+    const double zx = (double)vv.x + rr.x - yy.x, zy = (double)vv.y + rr.y - yy.y;
+    acc[0] += zx * zx + zy * zy;
+    const double ex = (double)gg.x - yy.x, ey = (double)gg.y - yy.y;
+    acc[1] += ex * ex + ey * ey;
+  }
+  block_sum<2>(acc, sh);
+  if (threadIdx.x == 0) {
+    part[((size_t)b * L2_SPLIT_MAX + c) * 2] = acc[0];
+    part[((size_t)b * L2_SPLIT_MAX + c) * 2 + 1] = acc[1];
+  }
+}
+
+__global__ __launch_bounds__(L2_THREADS) void k_l2_apply(const int64_t* __restrict__ blk_start, int S,
+                                                         const double* __restrict__ eps_arr,
+                                                         float2* __restrict__ v2, const float2* __restrict__ y,
+                                                         const float2* __restrict__ r2,
+                                                         const double* __restrict__ part,
+                                                         double* __restrict__ norm_res) {
+  __shared__ float fsh;
+  const int b = blockIdx.x / S, c = blockIdx.x % S;
+  if (threadIdx.x == 0) {
+    double z = 0.0, e = 0.0;
+    for (int k = 0; k < S; ++k) {
+      z += part[((size_t)b * L2_SPLIT_MAX + k) * 2];
+      e += part[((size_t)b * L2_SPLIT_MAX + k) * 2 + 1];
+    }
+    const double nz = sqrt(z);
+    fsh = nz > 0.0 ? (float)(1.0 - fmin(eps_arr[b] / nz, 1.0)) : 0.f;
+    if (c == 0) norm_res[b] = e;
+  }
+  __syncthreads();
+  const float f = fsh;
+  const int64_t a0 = blk_start[b], a1 = blk_start[b + 1];
+  const int64_t chunk = (a1 - a0 + S - 1) / S;
+  const int64_t lo = a0 + c * chunk, hi = lo + chunk < a1 ? lo + chunk : a1;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += L2_THREADS) {
+    const float2 vv = v2[i], rr = r2[i], yy = y[i];
+    v2[i] = make_float2((vv.x + rr.x - yy.x) * f, (vv.y + rr.y - yy.y) * f);
+  }
 }
 
 // ---- forward-backward projection onto the data ellipsoid ------------------
@@ -410,36 +537,6 @@ int fb_split_for(const fhs_data* D, int device) {
   return S;
 }
 
-// DR dual step: v2 <- sc(v2 + r2 - y, eps) (closed-form l2 prox), residual
-// ||gx - y||; sc(z, r) = z (1 - min(r/||z||, 1)).
-__global__ __launch_bounds__(FB_THREADS) void k_l2_dual(const int64_t* __restrict__ blk_start,
-                                                        const double* __restrict__ eps_arr,
-                                                        float2* __restrict__ v2,
-                                                        const float2* __restrict__ y,
-                                                        const float2* __restrict__ r2,
-                                                        const float2* __restrict__ gx,
-                                                        double* __restrict__ norm_res) {
-  __shared__ double sh[2 * FB_THREADS / 64];
-  const int64_t a = blk_start[blockIdx.x], b = blk_start[blockIdx.x + 1];
-  const double eps = eps_arr[blockIdx.x];
-  double acc[2] = {0.0, 0.0};
-  for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
-    const float2 vv = v2[i], rr = r2[i], yy = y[i], gg = gx[i];
-    const double zx = (double)vv.x + rr.x - yy.x, zy = (double)vv.y + rr.y - yy.y;
-    acc[0] += zx * zx + zy * zy;
-    const double ex = (double)gg.x - yy.x, ey = (double)gg.y - yy.y;
-    acc[1] += ex * ex + ey * ey;
-  }
-  block_sum<2>(acc, sh);
-  const double nz = sqrt(acc[0]);
-  const float f = nz > 0.0 ? (float)(1.0 - fmin(eps / nz, 1.0)) : 0.f;
-  for (int64_t i = a + threadIdx.x; i < b; i += FB_THREADS) {
-    const float2 vv = v2[i], rr = r2[i], yy = y[i];
-    v2[i] = make_float2((vv.x + rr.x - yy.x) * f, (vv.y + rr.y - yy.y) * f);
-  }
-  if (threadIdx.x == 0) norm_res[blockIdx.x] = acc[1];
-}
-
 // g2_l = G_l' (lam .* v2_l) (+ G_l (lam .* v2_l) in the DR form) for the
 // channels whose G keeps a CSC copy: one workgroup per adjoint work unit;
 // grid points past a channel's columns are zero.
@@ -517,6 +614,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_reduce(const GDev*
 
 void launch_data_adjoint(fhs_ctx* ctx, const fhs_data* D, const float2* v2, const float* lam, int dr) {
   const int L = D->L;
+  if (dr && D->stencil) {
+    const int units = (int)cdiv(D->maxM, 256);
+    k_dr_stencil<DR_ADJ><<<xcd_grid(L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, v2, D->No, lam, D->g2,
+                                                                      nullptr, L, units);
+    check_launch();
+    return;
+  }
   if (D->tile_grid > 0) {
     if (lam)
       k_data_adjoint_tiled<true><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, D->tile_lds, ctx->stream>>>(
@@ -656,19 +760,26 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
     Scope sc(ctx, "data_forward");
     const int units = (int)cdiv(D->maxM, 256);
     const unsigned grid = xcd_grid(L, units);
-    if (dr)
-      k_data_forward_dr<true><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2, L,
-                                                       units);
+    if (dr && D->stencil)
+      k_dr_stencil<DR_STATE><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old,
+                                                            D->r2, L, units);
+    else if (dr)
+      k_data_forward_dr<true><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old,
+                                                             D->r2, L, units);
     else
       launch_data_forward(ctx, D, gx_old, grid, units);
     check_launch();
   }
   {
     Scope sc(ctx, dr ? "l2_dual" : "fb_dual");
-    if (dr)
-      k_l2_dual<<<D->nblocks, FB_THREADS, 0, ctx->stream>>>(D->d_blk_start, D->d_eps, v2, y, D->r2, gx_old,
-                                                            D->d_norm_res);
-    else
+    if (dr) {
+      const unsigned g2grid = (unsigned)(D->nblocks * D->l2_split);
+      k_l2_partial<<<g2grid, L2_THREADS, 0, ctx->stream>>>(D->d_blk_start, D->l2_split, v2, y, D->r2, gx_old,
+                                                           D->l2_part);
+      check_launch();
+      k_l2_apply<<<g2grid, L2_THREADS, 0, ctx->stream>>>(D->d_blk_start, D->l2_split, D->d_eps, v2, y, D->r2,
+                                                         D->l2_part, D->d_norm_res);
+    } else
       k_fb_dual<<<D->nblocks * D->fb_split, FB_THREADS, FB_LDS, ctx->stream>>>(
           D->d_blk_start, D->d_eps, v2, y, pU, D->r2, proj, gx_old, max_it, min_it, tol, D->fb_split,
           fb_sync(D));
@@ -700,9 +811,12 @@ void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int
   FHS_HIP(hipMemsetAsync(gx_old, 0, D->Mtot * sizeof(float2), ctx->stream));
   const int units = (int)cdiv(D->maxM, 256);
   const unsigned grid = xcd_grid(D->L, units);
-  if (dr)
+  if (dr && D->stencil)
+    k_dr_stencil<DR_STATE><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2,
+                                                          D->L, units);
+  else if (dr)
     k_data_forward_dr<true><<<grid, 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No, lam, gx_old, D->r2,
-                                                     D->L, units);
+                                                           D->L, units);
   else
     launch_data_forward(ctx, D, gx_old, grid, units);
   check_launch();
@@ -729,8 +843,12 @@ void data_forward_dr(fhs_ctx* ctx, fhs_data* D, const float* x, const float* lam
   nufft_forward(ctx, D->A, x, D->L, D->Fx);
   Scope sc(ctx, "data_forward");
   const int units = (int)cdiv(D->maxM, 256);
-  k_data_forward_dr<false><<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No,
-                                                                          lam, y, nullptr, D->L, units);
+  if (D->stencil)
+    k_dr_stencil<DR_APPLY><<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx, D->No,
+                                                                           lam, y, nullptr, D->L, units);
+  else
+    k_data_forward_dr<false><<<xcd_grid(D->L, units), 256, 0, ctx->stream>>>(D->gdev, D->d_row_off, D->Fx,
+                                                                            D->No, lam, y, nullptr, D->L, units);
   check_launch();
 }
 
@@ -801,6 +919,9 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
       square = square && (G[l]->ncols == G[l]->M) && !G[l]->d.tiled;   // H of the DR form is not tiled
     }
     D->square = square;
+    bool sten = square;
+    for (int l = 0; l < L; ++l) sten = sten && G[l]->d.stencil != nullptr && G[l]->M == D->No;
+    D->stencil = sten;
     D->Mtot = D->row_off[L];
     D->blk_start.assign(block_start, block_start + nblocks + 1);
     D->blk_chan.assign(block_channel, block_channel + nblocks);
@@ -827,6 +948,14 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
     FHS_HIP(hipMemset(D->fb_sync, 0, (2 * (size_t)nblocks + 1) * sizeof(unsigned)));
     FHS_HIP(hipMalloc(&D->fb_part, (size_t)nblocks * (2 * FB_SPLIT_MAX * FB_PART + FB_SPLIT_MAX) * sizeof(double)));
     D->fb_split = fb_split_for(D, ctx->device);
+    // rows the DR kernels skip (inactive modes) must read as zero
+    FHS_HIP(hipMemset(D->r2, 0, (size_t)D->Mtot * sizeof(float2)));
+    if (square) {
+      int64_t longest = 0;
+      for (int b = 0; b < nblocks; ++b) longest = std::max(longest, D->blk_start[b + 1] - D->blk_start[b]);
+      D->l2_split = (int)std::min<int64_t>(L2_SPLIT_MAX, std::max<int64_t>(1, longest / (L2_THREADS * 16)));
+      FHS_HIP(hipMalloc(&D->l2_part, (size_t)nblocks * L2_SPLIT_MAX * 2 * sizeof(double)));
+    }
     *out = D;
   });
 }
@@ -835,7 +964,7 @@ int fhs_data_destroy(fhs_data* D) {
   if (!D) return FHS_OK;
   (void)hipSetDevice(D->device);
   void* ps[] = {D->gdev, D->d_row_off, D->d_blk_start, D->d_eps, D->d_norm_res,
-                D->d_sigma22, D->Fx, D->g2, D->r2, D->fb_sync, D->fb_part};
+                D->d_sigma22, D->Fx, D->g2, D->r2, D->fb_sync, D->fb_part, D->l2_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete D;

@@ -61,6 +61,13 @@ struct GDev {
   const int* prim_rows;
   const int4* fitems;
   int nfitems;
+  // holographic form (square, rows = columns = grid points): the full
+  // Hermitian stencil G'G of each active mode i, stencil[o * nact + i] for the
+  // (2Ky-1) x (2Kx-1) neighbourhood offsets o = (dy + Ky-1) + (2Ky-1)(dx + Kx-1),
+  // and act[p] = i for the active grid points (-1 elsewhere)
+  const float2* stencil;
+  const int* act;
+  int nact;
 };
 
 // the 16 complex slots of separable row m (8 x 16-byte loads)

@@ -21,8 +21,11 @@
 // (G[m,q] = 0 when q is outside m's window), in FP64, rows ascending.  A
 // count pass sizes row p of the lower triangle (q <= p) and column p of it
 // (q >= p); the host scans the counts; a fill pass recomputes the same sums
-// and writes the ELL rows and the CSC columns the data kernels read.  No
-// atomics: the result is deterministic.
+// and writes the ELL rows and the CSC columns that apply_direct_operator /
+// apply_adjoint_operator read, plus the full Hermitian stencil of every
+// active mode, (2Ky-1)(2Kx-1) values offset-major, that the DR data kernels
+// read: H + H' = G'G applied as a stencil, coalesced and without column
+// indices.  No atomics: the result is deterministic.
 #include <algorithm>
 #include <numeric>
 
@@ -113,10 +116,13 @@ __global__ __launch_bounds__(256) void k_holo_count(GDev g, HoloGeom h, int* __r
 
 // fill pass: ELL row p of tril(H) (slots [nlo_p, K) padded with column p and
 // value 0) and CSC column p (rows q >= p, values conj(H[p, q]) = H[q, p]);
-// the diagonal entry is halved in both
+// the diagonal entry is halved in both.  Active p also get their full
+// stencil row (every neighbourhood slot, zeros included, diagonal whole).
 __global__ __launch_bounds__(256) void k_holo_fill(GDev g, HoloGeom h, int K, const int* __restrict__ colptr,
                                                    int* __restrict__ ell_c, float2* __restrict__ ell_v,
-                                                   int* __restrict__ csc_r, float2* __restrict__ csc_v) {
+                                                   int* __restrict__ csc_r, float2* __restrict__ csc_v,
+                                                   const int* __restrict__ act, int nact,
+                                                   float2* __restrict__ stencil) {
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * HOLO_WAVES + (threadIdx.x >> 6);
   if (p >= h.No) return;
@@ -125,6 +131,12 @@ __global__ __launch_bounds__(256) void k_holo_fill(GDev g, HoloGeom h, int K, co
   bool nz[HOLO_SLOTS];
   double d;
   holo_point(g, h, (int)p, lane, val, q, nz, d);
+  const int ia = act[p];
+  if (ia >= 0) {
+#pragma unroll
+    for (int s = 0; s < HOLO_SLOTS; ++s)
+      if (q[s] >= 0) stencil[(size_t)(lane + 64 * s) * nact + ia] = val[s];
+  }
   const int c1 = colptr[p + 1];
   const unsigned long long below = (1ull << lane) - 1ull;
   int lo = 0, up = colptr[p];
@@ -234,6 +246,16 @@ int fhs_gmat_create_holographic(fhs_ctx* ctx, const fhs_gmat* G, fhs_gmat** out)
         modes += nlo[p] > 0;
       }
       FHS_CHECK(nnz == nnz_up, "gmat_create_holographic: lower/upper counts disagree");
+      FHS_CHECK(modes > 0, "gmat_create_holographic: G touches no grid point");
+      std::vector<int> act(No);
+      {
+        int i = 0;
+        for (int64_t p = 0; p < No; ++p) act[p] = nlo[p] > 0 ? i++ : -1;
+      }
+      int* d_act = dev_upload(act);
+      H->mem[8] = d_act;
+      float2* sten = dev_alloc<float2>((size_t)h.ny * h.nx * modes);
+      H->mem[5] = sten;
       int* ell_c = dev_alloc<int>((size_t)K * No);
       H->mem[0] = ell_c;
       float2* ell_v = dev_alloc<float2>((size_t)K * No);
@@ -244,7 +266,8 @@ int fhs_gmat_create_holographic(fhs_ctx* ctx, const fhs_gmat* G, fhs_gmat** out)
       H->mem[3] = csc_r;
       float2* csc_v = dev_alloc<float2>(nnz);
       H->mem[4] = csc_v;
-      k_holo_fill<<<blocks, 64 * HOLO_WAVES, 0, ctx->stream>>>(g, h, K, d_colptr, ell_c, ell_v, csc_r, csc_v);
+      k_holo_fill<<<blocks, 64 * HOLO_WAVES, 0, ctx->stream>>>(g, h, K, d_colptr, ell_c, ell_v, csc_r, csc_v,
+                                                                d_act, (int)modes, sten);
       check_launch();
       const std::vector<int> units = adjoint_units(colptr);
       H->mem[6] = dev_upload(units);
@@ -265,9 +288,16 @@ int fhs_gmat_create_holographic(fhs_ctx* ctx, const fhs_gmat* G, fhs_gmat** out)
       d.csc_vals = csc_v;
       d.ucol = (const int*)H->mem[6];
       d.nunits = (int)units.size() - 1;
+      d.Kty = h.Kty;
+      d.Ktx = h.Ktx;
+      d.Ky = h.Ky;
+      d.Kx = h.Kx;
+      d.stencil = sten;
+      d.act = d_act;
+      d.nact = (int)modes;
     } catch (...) {
-      hipStreamSynchronize(ctx->stream);
-      for (void* p : tmp) hipFree(p);
+      (void)hipStreamSynchronize(ctx->stream);
+      for (void* p : tmp) (void)hipFree(p);
       if (H) fhs_gmat_destroy(H);
       throw;
     }

@@ -263,3 +263,35 @@ def test_dr_hypersara_matches_oracle(ctx):
           f'oracle {snr(prob.X0, x_ref):.3f}')
     assert r < 1e-3, r
     assert abs(snr(prob.X0, x_gpu) - snr(prob.X0, x_ref)) < 0.1
+
+
+def test_dr_dual_step_full_size(ctx):
+    """512 x 512 x 2 with 160,000 visibilities per channel: one DR dual step
+    from the zero state (grid-sized blocks of 1,048,576 rows, the dual split
+    over 256 workgroups each) against its closed form evaluated in FP64 from
+    the DR operator's own output: v2 = sc(2 gx - y, eps), ||gx - y||."""
+    import fhs
+    from fhs import solvers, synth
+    from fhs.holographic import HolographicMeasurement
+    prob = synth.make_problem(512, 512, 2, 160000, 1, seed=6)
+    meas = solvers.Measurement(ctx, prob, precond=False)
+    dm = HolographicMeasurement(meas, noise_seed=3)
+    L, No = 2, dm.forms[0].No
+    x = torch.from_numpy(np.random.default_rng(1).random((L, 512, 512)).astype(np.float32)).cuda()
+    gx = dm.forward(x).to(torch.complex128)
+    y = dm.y.to(torch.complex128)
+    eps = np.array([0.5 * torch.linalg.vector_norm(2 * gx[l * No:(l + 1) * No] - y[l * No:(l + 1) * No]).item()
+                    for l in range(L)])
+    gx_old = dm.D.init_fx_old(torch.zeros_like(x), dm.Lambda, True)
+    assert torch.count_nonzero(gx_old).item() == 0
+    v2d, _, gx_old, _, nrd, _, _ = fhs.update_dual_data_fidelity(
+        dm.D.zeros(), dm.y, x, gx_old, dm.D.zeros(), dm.plan, dm.plan, dm.D, None, None, eps, 20, 1, 1e-8,
+        np.full(L, 0.5, np.float32), flag_dimensionality_reduction=True, Lambda=dm.Lambda)
+    assert torch.equal(gx_old, dm.forward(x))
+    for l in range(L):
+        s = slice(l * No, (l + 1) * No)
+        z = 2 * gx[s] - y[s]
+        nz = torch.linalg.vector_norm(z).item()
+        want = z * (1 - min(eps[l] / nz, 1))
+        assert rel(v2d[s].cpu().numpy(), want.cpu().numpy()) < TOL, l
+        assert abs(nrd[l] - torch.linalg.vector_norm(gx[s] - y[s]).item()) < TOL * nrd[l], l
