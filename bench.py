@@ -53,18 +53,22 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--config', choices=('hs512', 'fhs1024', 'sara256'), default='hs512',
+    ap.add_argument('--config', choices=('hs512', 'hs512dr', 'fhs1024', 'sara256'), default='hs512',
                     help='hs512: HyperSARA 512x512x16 (the headline workload); fhs1024: one 16-channel spectral '
                          'sub-cube of Faceted HyperSARA 1024x1024x32 (C = 2), 4x4 nuclear-norm facets with 25%% '
                          'overlap, 300k-500k visibilities per channel; sara256: SARA 256x256, 65,536 '
                          'visibilities, 2 blocks, 300 iterations, against the FP64 oracle run to the same end '
-                         '(the CPU-runnable case: its SNR is the reference)')
+                         '(the CPU-runnable case: its SNR is the reference); hs512dr: the hs512 problem in its '
+                         'dimensionality-reduced form (holographic H = tril(G\'G) per channel, no preconditioning, '
+                         'imaging/imaging.m:619-631), its own converged SNR the reference')
     a = ap.parse_args()
     if a.config == 'fhs1024':
         a.size, a.channels, a.vis = 1024, 16, 500000
         a.no_cpu = True     # one FP64 oracle iteration of this size takes minutes
     if a.config == 'sara256':
         a.size, a.channels, a.vis = 256, 1, 65536
+    if a.config == 'hs512dr':
+        a.no_cpu = True     # the FP64 oracle's sparse G'G of 16 channels takes minutes
     return a
 
 
@@ -102,6 +106,23 @@ def algorithmic_bytes(meas, sara_s):
         'primal': 4 * L * N * 6,
         'fb_dual': 44 * sum(M),
     }
+
+
+def algorithmic_bytes_dr(meas, sara_s):
+    """The DR form's per-launch algorithmic bytes: every active mode's full
+    Hermitian stencil ((2Ky-1)(2Kx-1) complex values) read once per apply, the
+    grid vectors once; FFT and prior kernels as in the standard form."""
+    out = algorithmic_bytes(meas.meas, sara_s)
+    nf = meas.meas.nufft
+    No = nf.No
+    ns = (2 * nf.Ky - 1) * (2 * nf.Kx - 1)
+    W = [f.n_modes for f in meas.forms]
+    st = sum(8 * ns * w for w in W)
+    out['data_forward'] = st + sum(8 * No + 28 * w for w in W)     # Fx in; Lambda, gx_old r/w, r2 on W
+    out['data_adjoint'] = st + len(W) * 28 * No                    # v2, Lambda in; g2 out
+    out['l2_dual'] = 64 * len(W) * No                              # two passes over v2, r2, y (+ gx), v2 out
+    del out['fb_dual']
+    return out
 
 
 def sara_against_oracle(args, solvers, meas, prob, son, nu2, n_iter=300):
@@ -185,9 +206,16 @@ def main():
     else:
         prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
     kind = 'fhs' if facets is not None else ('sara' if args.config == 'sara256' else 'hs')
-    meas = solvers.Measurement(ctx, prob, precond=True)
-    son = meas.op_norm(False)
-    nu2 = meas.op_norm(True)
+    dr = args.config == 'hs512dr'
+    if dr:
+        from fhs.holographic import HolographicMeasurement
+        meas = HolographicMeasurement(solvers.Measurement(ctx, prob, precond=False))
+        son = meas.op_norm()
+        nu2 = son
+    else:
+        meas = solvers.Measurement(ctx, prob, precond=True)
+        son = meas.op_norm(False)
+        nu2 = meas.op_norm(True)
     setup_s = time.perf_counter() - t_setup
     param = {'squared_operator_norm': son, 'nu2': nu2, 'reweighting_max_iter': 3}
 
@@ -219,7 +247,7 @@ def main():
     eng.set_concurrent(True)
     rep = ctx.timing_report()
     ctx.timing_reset()
-    alg = algorithmic_bytes(meas, eng.sara.s)
+    alg = algorithmic_bytes_dr(meas, eng.sara.s) if dr else algorithmic_bytes(meas, eng.sara.s)
     per_iter_ms = {k: v[0] / n_timed for k, v in rep.items()}
     dominant = max((k for k in rep if k in alg), key=lambda k: rep[k][0])
     avg_ms = rep[dominant][0] / rep[dominant][1]
@@ -275,7 +303,7 @@ def main():
                            'converged': plain['converged'], 'reweights': plain['reweights'],
                            'reach': p_reach, 'reach_s_each': p_all}}
         wpath = warmstart.default_weights_path()
-        if os.path.exists(wpath):
+        if os.path.exists(wpath) and not dr:
             ws = warmstart.WarmStart.load(ctx, wpath)
             ws.predict(warmstart.dirty_cube(meas))      # one untimed call: workspaces and code objects
             torch.cuda.synchronize()
@@ -344,20 +372,26 @@ def main():
         'data': 'synthetic',
         'config': {'workload': (f'SARA {args.size}x{args.size}, {args.vis} visibilities, 2 blocks'
                                 if args.config == 'sara256' else
+                                f'HyperSARA {args.size}x{args.size}x{args.channels}, {args.vis} base vis/channel, '
+                                f'10-30% flags, dimensionality-reduced (holographic) form, one block/channel'
+                                if dr else
                                 f'HyperSARA {args.size}x{args.size}x{args.channels}, '
                                 f'{args.vis} base vis/channel, 10-30% flags, 4 blocks/channel')
                    if facets is None else
                    (f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 2 spectral sub-cubes '
                     f'of {args.size}x{args.size}x{2 * args.channels}), 4x4 nuclear-norm facets, 25% overlap, '
                     f'{int(args.vis * 0.6)}-{args.vis} vis/channel, 4 blocks/channel'),
-                   'Mtot': int(meas.D.Mtot), 'nnz_G_total': int(sum(g.nnz for g in meas.G)),
+                   'Mtot': int(meas.D.Mtot),
+                   'nnz_G_total': int(sum(g.nnz for g in (meas.meas.G if dr else meas.G))),
+                   **({'active_modes': int(sum(f.n_modes for f in meas.forms)),
+                       'nnz_H_lower_total': int(sum(f.H.nnz for f in meas.forms))} if dr else {}),
                    'setup_s': round(setup_s, 2)},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1),
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
                      'traffic': traffic, 'avg_launch_ms': round(avg_ms, 5),
                      'algorithmic_bytes_per_launch': int(alg[dominant]),
-                     'survey_formula_bytes_per_launch': int(sum(8 * g.nnz + 12 * g.M + 8 * meas.nufft.No
-                                                                for g in meas.G))},
+                     'survey_formula_bytes_per_launch': None if dr else
+                     int(sum(8 * g.nnz + 12 * g.M + 8 * meas.nufft.No for g in meas.G))},
         'kernels': kernels,
         'solve': solve,
         'cpu_baseline': cpu,
