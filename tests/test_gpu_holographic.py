@@ -38,13 +38,14 @@ def to_dev(a):
 
 # 16 x 16 has a sparse, ragged coverage of its 32 x 32 grid (inactive modes,
 # rows with few entries); 64 x 64 with 30,000 visibilities covers most of it
-@pytest.mark.parametrize('Ny,Nx,M', [(16, 16, 300), (32, 64, 4000), (64, 64, 30000)])
-def test_holographic_matches_oracle(ctx, Ny, Nx, M):
+@pytest.mark.parametrize('Ny,Nx,M,K', [(16, 16, 300, (7, 7)), (32, 64, 4000, (7, 7)), (64, 64, 30000, (7, 7)),
+                                       (32, 32, 3000, (5, 5)), (32, 32, 3000, (6, 4))])
+def test_holographic_matches_oracle(ctx, Ny, Nx, M, K):
     import fhs
     from fhs.holographic import Holographic
     rng = np.random.default_rng(M)
     u, v = uv(rng, M)
-    op = NufftOracle(u, v, Ny, Nx)
+    op = NufftOracle(u, v, Ny, Nx, Ky=K[0], Kx=K[1])
     G = fhs.GMatrix.from_separable(ctx, op.anchor, op.cy_sep, op.cx_sep, op.Kty, op.Ktx)
     Gs = op.G.tocsr()
     y, n = cplx(rng, M), cplx(rng, M) / np.sqrt(2)
@@ -87,8 +88,8 @@ def test_holographic_build_is_deterministic(ctx):
     assert torch.equal(fhs.apply_adjoint_operator(f, a.H), fhs.apply_adjoint_operator(f, b.H))
 
 
-@pytest.mark.parametrize('L', [1, 2])
-def test_dr_dual_step_on_built_data(ctx, L):
+@pytest.mark.parametrize('L,K', [(1, (7, 7)), (2, (7, 7)), (2, (6, 4))])
+def test_dr_dual_step_on_built_data(ctx, L, K):
     """update_dual_data_fidelity's DR branch on the builder's H, Lambda and
     gridded data against the oracle's DR branch on the oracle's (compact |W|
     vectors), over consecutive iterations; eps from the gridded noise."""
@@ -100,7 +101,7 @@ def test_dr_dual_step_on_built_data(ctx, L):
     ops, forms, Hs, Ws, lams, ygs, eps, yds = [], [], [], [], [], [], [], []
     for l in range(L):
         u, v = uv(rng, M, 0.8 + 0.1 * l)
-        op = NufftOracle(u, v, Ny, Nx)
+        op = NufftOracle(u, v, Ny, Nx, Ky=K[0], Kx=K[1])
         x0 = rng.random((Ny, Nx))
         y = op.G @ op.A(x0) + 0.05 * cplx(rng, M)
         n = cplx(rng, M) / np.sqrt(2)
