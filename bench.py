@@ -53,10 +53,12 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--config', choices=('hs512', 'hs512dr', 'fhs1024', 'sara256'), default='hs512',
+    ap.add_argument('--config', choices=('hs512', 'hs512dr', 'fhs1024', 'fhs2048', 'sara256'), default='hs512',
                     help='hs512: HyperSARA 512x512x16 (the headline workload); fhs1024: one 16-channel spectral '
                          'sub-cube of Faceted HyperSARA 1024x1024x32 (C = 2), 4x4 nuclear-norm facets with 25%% '
-                         'overlap, 300k-500k visibilities per channel; sara256: SARA 256x256, 65,536 '
+                         'overlap, 300k-500k visibilities per channel; fhs2048: one 16-channel spectral sub-cube '
+                         'of Faceted HyperSARA 2048x2048x64 (C = 4), 6x6 facets with 50%% overlap, 1M visibilities '
+                         'per channel, no w-term; sara256: SARA 256x256, 65,536 '
                          'visibilities, 2 blocks, 300 iterations, against the FP64 oracle run to the same end '
                          '(the CPU-runnable case: its SNR is the reference); hs512dr: the hs512 problem in its '
                          'dimensionality-reduced form (holographic H = tril(G\'G) per channel, no preconditioning, '
@@ -65,6 +67,9 @@ def parse():
     if a.config == 'fhs1024':
         a.size, a.channels, a.vis = 1024, 16, 500000
         a.no_cpu = True     # one FP64 oracle iteration of this size takes minutes
+    if a.config == 'fhs2048':
+        a.size, a.channels, a.vis = 2048, 16, 1000000
+        a.no_cpu = True
     if a.config == 'sara256':
         a.size, a.channels, a.vis = 256, 1, 65536
     if a.config == 'hs512dr':
@@ -201,6 +206,11 @@ def main():
         prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed,
                                   flag_range=(0.0, 0.4))
         facets = facet_geometry(args.size, args.size, 4, 4, (0.25, 0.25))
+    elif args.config == 'fhs2048':
+        from fhs.lowrank import facet_geometry
+        prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed,
+                                  flag_range=None)
+        facets = facet_geometry(args.size, args.size, 6, 6, (0.5, 0.5))
     elif args.config == 'sara256':
         prob = synth.make_problem(args.size, args.size, 1, args.vis, 2, seed=args.seed)
     else:
@@ -378,7 +388,11 @@ def main():
                                 f'HyperSARA {args.size}x{args.size}x{args.channels}, '
                                 f'{args.vis} base vis/channel, 10-30% flags, 4 blocks/channel')
                    if facets is None else
-                   (f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 2 spectral sub-cubes '
+                   (f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 4 spectral sub-cubes '
+                    f'of {args.size}x{args.size}x{4 * args.channels}), 6x6 nuclear-norm facets, 50% overlap, '
+                    f'{args.vis} vis/channel, no w-term, 4 blocks/channel'
+                    if args.config == 'fhs2048' else
+                    f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 2 spectral sub-cubes '
                     f'of {args.size}x{args.size}x{2 * args.channels}), 4x4 nuclear-norm facets, 25% overlap, '
                     f'{int(args.vis * 0.6)}-{args.vis} vis/channel, 4 blocks/channel'),
                    'Mtot': int(meas.D.Mtot),

@@ -128,6 +128,25 @@ constexpr int nt_strided() {
   return b_strided<N>() * N / 4 > 1024 ? 1024 : (b_strided<N>() * N / 4 < 64 ? 64 : b_strided<N>() * N / 4);
 }
 
+// x-pass workgroup -> first grid row.  Workgroups are dispatched to the 8
+// XCDs round-robin (linear id mod 8; gridDim.x is a multiple of 8 here, so
+// blockIdx.x mod 8).  When B rows are fewer than a 128-byte line (G = 16 / B
+// workgroups share each line of every grid column), workgroups b, b + 8, ...,
+// b + 8 (G - 1) take G adjacent row groups, so the line is fetched into one
+// XCD's L2 instead of G.
+template <int N>
+__device__ __forceinline__ int xpass_row0() {
+  constexpr int B = b_strided<N>();
+  constexpr int G = 16 / B;
+  const int b = blockIdx.x;
+  if constexpr (G == 1) {
+    return b * B;
+  } else {
+    if (gridDim.x % (8 * G)) return b * B;
+    return ((b / (8 * G)) * (8 * G) + (b % 8) * G + (b / 8) % G) * B;
+  }
+}
+
 // (1) forward y-pass
 template <int N>
 __global__ __launch_bounds__(nt_contig<N>()) void k_fwd_y(const float* __restrict__ x,
@@ -169,7 +188,7 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restr
   constexpr int NT = nt_strided<N>();
   __shared__ float2 s[N * B];
   __shared__ float2 tws[TW_LDS<N>];
-  const int ky0 = blockIdx.x * B, l = blockIdx.y;
+  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
   const float2* twp = stage_twiddles<N, NT>(tw, tws);
   const float2* Tl = T + (size_t)l * Nx * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
@@ -200,7 +219,7 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
   constexpr int NT = nt_strided<N>();
   __shared__ float2 s[N * B];
   __shared__ float2 tws[TW_LDS<N>];
-  const int ky0 = blockIdx.x * B, l = blockIdx.y;
+  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
   const float2* twp = stage_twiddles<N, NT>(tw, tws);
   const float2* gl = g + (size_t)l * N * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
