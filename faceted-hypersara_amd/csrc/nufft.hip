@@ -11,7 +11,7 @@
 //   adjoint : (1') x-pass inverse keeping the first Nx outputs -> T;
 //             (2') y-pass inverse, keep Ny outputs, * sn, real part, * scale.
 // The x-pass reads and writes B consecutive complex values per grid column
-// (B = 16 -> 128-byte segments, B = 8 -> 64) and keeps B transforms interleaved in LDS
+// (B = 16 -> 128-byte segments, B = 8 -> 64, B = 4 -> 32) and keeps B transforms interleaved in LDS
 // (s[i*B + b]) so that consecutive lanes touch consecutive LDS words.
 // FFTs are radix-4 Stockham (autosort) with one radix-2 stage when log2 n is
 // odd; twiddles come from an FP64-computed table exp(-2 pi i k/n).
@@ -118,11 +118,13 @@ __device__ __forceinline__ const float2* stage_twiddles(const float2* __restrict
 
 template <int N>
 constexpr int nt_contig() { return N / 4 < 64 ? 64 : (N / 4 > 1024 ? 1024 : N / 4); }
-// grid rows per x-pass workgroup: at least 8 (64-byte global segments per
-// column) up to N = 2048 (128 KB of LDS for the rows, one workgroup per CU),
-// 4 at N = 4096
+// grid rows per x-pass workgroup: 16 up to N = 512, 8 at N = 1024 (two
+// workgroups per CU), 4 from N = 2048 (64 KB of rows plus the twiddles, two
+// workgroups per CU; 128 KB and one per CU at N = 4096, where 2 rows, 16-byte
+// segments, measured slower).  Segments below 128 bytes are merged in one
+// XCD's L2 by xpass_row0.
 template <int N>
-constexpr int b_strided() { return N >= 4096 ? 4 : (N >= 1024 ? 8 : 16); }
+constexpr int b_strided() { return N >= 2048 ? 4 : (N >= 1024 ? 8 : 16); }
 template <int N>
 constexpr int nt_strided() {
   return b_strided<N>() * N / 4 > 1024 ? 1024 : (b_strided<N>() * N / 4 < 64 ? 64 : b_strided<N>() * N / 4);
