@@ -36,6 +36,11 @@ METRIC = 'time-to-converged cube (s) at ref SNR−0.1 dB, chosen vs plain form; 
 PEAK_HBM_GBS = 8000.0
 
 
+def log(msg):
+    """One progress line on stderr per finished stage (stdout keeps the JSON line)."""
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -227,6 +232,7 @@ def main():
         son = meas.op_norm(False)
         nu2 = meas.op_norm(True)
     setup_s = time.perf_counter() - t_setup
+    log(f'setup {setup_s:.1f} s')
     param = {'squared_operator_norm': son, 'nu2': nu2, 'reweighting_max_iter': 3}
 
     # ---- throughput: steps through the hipGraph engine --------------------
@@ -245,6 +251,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     iters_per_s = args.steps / elapsed
+    log(f'{args.steps} timed iterations: {iters_per_s:.2f} iterations/s')
 
     # ---- live per-kernel timing (HIP events on the engine stream) ---------
     # kernels in issue order on one stream, so each duration is its own
@@ -287,6 +294,7 @@ def main():
         # same deterministic sequence, each stopped once it reaches ref - 0.1 dB
         _, plain = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds)
         ref = plain['snr'][-1]
+        log(f'reference solve: {plain["iterations"]} iterations, {plain["wall"]:.1f} s, SNR {ref:.3f} dB')
         tols = (0.1, 0.5, 1.0, 2.0, 3.0)
 
         def timed(**kw):
@@ -306,6 +314,7 @@ def main():
             return out, every
 
         p_reach, p_all = reach(timed())
+        log('timed plain solves done')
         solve = {'ref_snr_db': round(ref, 3), 'ref_definition': 'SNR of the plain run at convergence '
                  '(3 reweighting steps, reference stopping rules); SNR = 20 log10(||X0||/||X0 - X||)',
                  'timing': f'median of {max(1, args.solve_repeats)} solves after the full reference solve',
@@ -326,7 +335,9 @@ def main():
                                    torch.linalg.vector_norm(X0d - x0.double()).item())
             _, warm = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, x_init=x0, weights_from_init=False,
                                      time_limit=args.solve_seconds)
+            log(f'warm-start solve: {warm["iterations"]} iterations')
             w_reach, w_all = reach(timed(x_init=x0, weights_from_init=False), t_pred)
+            log('timed warm-start solves done')
             solve['warm_start'] = {'network': os.path.relpath(wpath, ROOT), 'predict_s': round(t_pred, 4),
                                    'x_init_snr_db': round(x0_snr, 3),
                                    'iterations': warm['iterations'], 'final_snr_db': round(warm['snr'][-1], 3),
