@@ -103,16 +103,19 @@ __device__ __forceinline__ void fft_lds(float2* s, const float2* tw) {
   }
 }
 
-// The radix-4 stages read twiddles tw[r k tstride] < 3N/4: up to N = 2048
-// they are staged in LDS once per workgroup (read behind the data loads'
-// barrier) instead of a cache round trip in every stage; at N = 4096 the
-// table would halve the x-pass occupancy, so the stages read it from cache.
+// The radix-4 stages read twiddles tw[r k tstride] < 3N/4.  They are staged
+// in LDS once per workgroup (read behind the data loads' barrier) instead of
+// a cache round trip in every stage.  The y-passes (one column, N x 8 bytes
+// of LDS) always stage them; the x-passes up to N = 2048 only, since at
+// N = 4096 the table would halve their occupancy.
 template <int N>
 constexpr int TW_LDS = (N < 4 || N > 2048) ? 1 : 3 * N / 4;
-template <int N, int NT>
+template <int N>
+constexpr int TW_LDS_Y = N < 4 ? 1 : 3 * N / 4;
+template <int N, int NT, int TWN>
 __device__ __forceinline__ const float2* stage_twiddles(const float2* __restrict__ tw, float2* tws) {
-  if constexpr (N > 2048) return tw;
-  for (int i = threadIdx.x; i < TW_LDS<N>; i += NT) tws[i] = tw[i];
+  if constexpr (TWN == 1) return tw;
+  for (int i = threadIdx.x; i < TWN; i += NT) tws[i] = tw[i];
   return tws;
 }
 
@@ -158,9 +161,9 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_fwd_y(const float* __restric
                                                          float2* __restrict__ T) {
   constexpr int NT = nt_contig<N>();
   __shared__ float2 s[N];
-  __shared__ float2 tws[TW_LDS<N>];
+  __shared__ float2 tws[TW_LDS_Y<N>];
   const int ix = blockIdx.x, l = blockIdx.y;
-  const float2* twp = stage_twiddles<N, NT>(tw, tws);
+  const float2* twp = stage_twiddles<N, NT, TW_LDS_Y<N>>(tw, tws);
   const float* xc = x + ((size_t)l * Nx + ix) * Ny;
   const float scx = sx[ix];
   constexpr int PER = (N + NT - 1) / NT;
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restr
   __shared__ float2 s[N * B];
   __shared__ float2 tws[TW_LDS<N>];
   const int ky0 = xpass_row0<N>(), l = blockIdx.y;
-  const float2* twp = stage_twiddles<N, NT>(tw, tws);
+  const float2* twp = stage_twiddles<N, NT, TW_LDS<N>>(tw, tws);
   const float2* Tl = T + (size_t)l * Nx * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
   float2 v[PER];
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
   __shared__ float2 s[N * B];
   __shared__ float2 tws[TW_LDS<N>];
   const int ky0 = xpass_row0<N>(), l = blockIdx.y;
-  const float2* twp = stage_twiddles<N, NT>(tw, tws);
+  const float2* twp = stage_twiddles<N, NT, TW_LDS<N>>(tw, tws);
   const float2* gl = g + (size_t)l * N * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
   float2 v[PER];
@@ -252,9 +255,9 @@ __global__ __launch_bounds__(nt_contig<N>()) void k_adj_y(const float2* __restri
                                                          float* __restrict__ out) {
   constexpr int NT = nt_contig<N>();
   __shared__ float2 s[N];
-  __shared__ float2 tws[TW_LDS<N>];
+  __shared__ float2 tws[TW_LDS_Y<N>];
   const int ix = blockIdx.x, l = blockIdx.y;
-  const float2* twp = stage_twiddles<N, NT>(tw, tws);
+  const float2* twp = stage_twiddles<N, NT, TW_LDS_Y<N>>(tw, tws);
   const float2* Tc = T + ((size_t)l * Nx + ix) * N;
   constexpr int PER = (N + NT - 1) / NT;
   float2 v[PER];
