@@ -245,6 +245,113 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
   }
 }
 
+// Split x-passes for N >= X_SPLIT_MIN when the image holds at most half the
+// grid columns (Nx <= N / 2, true of every 2x-oversampled grid).  With
+// H = N / 2 and W = exp(-2 pi i / N):
+//   forward : only x[0 .. H) is non-zero, so
+//             X[2m] = FFT_H(x)[m],  X[2m + 1] = FFT_H(W^n x[n])[m];
+//   adjoint : only y[0 .. H) is kept, so
+//             y[n] = IFFT_H(g[2m])[n] + conj(W^n) IFFT_H(g[2m + 1])[n].
+// The two H-point transforms run one after the other in the same H x B
+// rows of LDS (64 KB at N = 4096 instead of 128 KB: two workgroups per CU),
+// the H-point twiddles being every other entry of the N-point table.
+constexpr int X_SPLIT_MIN = 4096;
+
+template <int N>
+__global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x_split(const float2* __restrict__ T, int Kty,
+                                                                int Nx, const float2* __restrict__ tw,
+                                                                float2* __restrict__ Fx) {
+  constexpr int H = N / 2;
+  constexpr int B = b_strided<N>();
+  constexpr int NT = nt_strided<N>();
+  static_assert(H * B % NT == 0, "split x-pass: whole rows per thread");
+  __shared__ float2 s[H * B];
+  __shared__ float2 tws[3 * H / 4];
+  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
+  for (int i = threadIdx.x; i < 3 * H / 4; i += NT) tws[i] = tw[2 * i];
+  const float2* Tl = T + (size_t)l * Nx * Kty;
+  constexpr int PER = H * B / NT;
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    const int b = t % B, i = t / B;
+    v[j] = i < Nx ? Tl[(size_t)i * Kty + ky0 + b] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s[threadIdx.x + j * NT] = v[j];
+  __syncthreads();
+  fft_lds<H, B, NT, false>(s, tws);
+  float2* F = Fx + (size_t)l * N * Kty;
+  // each thread reads back, then overwrites, only its own LDS slots
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    const int b = t % B, k = t / B;
+    F[(size_t)(2 * k) * Kty + ky0 + b] = s[t];
+    s[t] = cmul(v[j], tw[k]);
+  }
+  __syncthreads();
+  fft_lds<H, B, NT, false>(s, tws);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    const int b = t % B, k = t / B;
+    F[(size_t)(2 * k + 1) * Kty + ky0 + b] = s[t];
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(nt_strided<N>()) void k_adj_x_split(const float2* __restrict__ g, int Kty,
+                                                                int Nx, const float2* __restrict__ tw,
+                                                                float2* __restrict__ T) {
+  constexpr int H = N / 2;
+  constexpr int B = b_strided<N>();
+  constexpr int NT = nt_strided<N>();
+  static_assert(H * B % NT == 0, "split x-pass: whole rows per thread");
+  __shared__ float2 s[H * B];
+  __shared__ float2 tws[3 * H / 4];
+  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
+  for (int i = threadIdx.x; i < 3 * H / 4; i += NT) tws[i] = tw[2 * i];
+  const float2* gl = g + (size_t)l * N * Kty;
+  constexpr int PER = H * B / NT;
+  float2 v[PER], e[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    v[j] = gl[(size_t)(2 * (t / B)) * Kty + ky0 + t % B];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s[threadIdx.x + j * NT] = v[j];
+  // the odd rows are loaded while the even half is transformed
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    v[j] = gl[(size_t)(2 * (t / B) + 1) * Kty + ky0 + t % B];
+  }
+  __syncthreads();
+  fft_lds<H, B, NT, true>(s, tws);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    e[j] = s[t];
+    s[t] = v[j];
+  }
+  __syncthreads();
+  fft_lds<H, B, NT, true>(s, tws);
+  float2* Tl = T + (size_t)l * Nx * Kty;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int t = threadIdx.x + j * NT;
+    const int b = t % B, n = t / B;
+    if (n < Nx) {
+      float2 w = tw[n];
+      w.y = -w.y;
+      Tl[(size_t)n * Kty + ky0 + b] = cadd(e[j], cmul(s[t], w));
+    }
+  }
+}
+
 // (2') adjoint y-pass: N = Kty, keep Ny outputs, real part, scale
 template <int N>
 __global__ __launch_bounds__(nt_contig<N>()) void k_adj_y(const float2* __restrict__ T,
@@ -302,12 +409,24 @@ struct RunFwdY {
 template <int N>
 struct RunFwdX {
   static void run(hipStream_t st, int Kty, int L, const float2* T, const fhs_nufft* p, float2* F) {
+    if constexpr (N >= X_SPLIT_MIN) {
+      if (2 * p->Nx <= N) {
+        k_fwd_x_split<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(T, Kty, p->Nx, p->twx, F);
+        return;
+      }
+    }
     k_fwd_x<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(T, Kty, p->Nx, p->twx, F);
   }
 };
 template <int N>
 struct RunAdjX {
   static void run(hipStream_t st, int Kty, int L, const float2* g, const fhs_nufft* p, float2* T) {
+    if constexpr (N >= X_SPLIT_MIN) {
+      if (2 * p->Nx <= N) {
+        k_adj_x_split<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(g, Kty, p->Nx, p->twx, T);
+        return;
+      }
+    }
     k_adj_x<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(g, Kty, p->Nx, p->twx, T);
   }
 };

@@ -63,13 +63,16 @@ def test_direct_and_adjoint(ctx, Ny, Nx, M, layout):
     assert np.array_equal(g2, g3)
 
 
-@pytest.mark.parametrize('Ny,Nx', [(8, 8), (16, 32), (64, 64), (100, 60), (256, 256), (512, 512), (1024, 512),
-                                   (512, 1024), (256, 2048), (64, 2048), (2048, 64)])
-def test_scaled_fourier(ctx, Ny, Nx):
+# Ktx = 4096 with Nx <= 2048 takes the split x-passes (two 2048-point
+# transforms); (16, 4096) without x oversampling takes the full 4096-point ones
+@pytest.mark.parametrize('Ny,Nx,ox', [(8, 8, 2), (16, 32, 2), (64, 64, 2), (100, 60, 2), (256, 256, 2),
+                                      (512, 512, 2), (1024, 512, 2), (512, 1024, 2), (256, 2048, 2), (64, 2048, 2),
+                                      (2048, 64, 2), (16, 4096, 1)])
+def test_scaled_fourier(ctx, Ny, Nx, ox):
     import fhs
     rng = np.random.default_rng(Ny + Nx)
-    op = NufftOracle(np.zeros(1), np.zeros(1), Ny, Nx)
-    Kty, Ktx = 1 << int(np.ceil(np.log2(2 * Ny))), 1 << int(np.ceil(np.log2(2 * Nx)))
+    op = NufftOracle(np.zeros(1), np.zeros(1), Ny, Nx, ox=ox)
+    Kty, Ktx = 1 << int(np.ceil(np.log2(2 * Ny))), 1 << int(np.ceil(np.log2(ox * Nx)))
     assert (Kty, Ktx) == (op.Kty, op.Ktx) or Ny == 100
     if Ny == 100:
         pytest.skip('grid must be a power of two')
