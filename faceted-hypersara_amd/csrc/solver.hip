@@ -34,6 +34,8 @@ void data_step(fhs_ctx* ctx, fhs_data* D, float2* v2, const float2* y, const flo
 void data_set_sigma22(fhs_ctx* ctx, fhs_data* D, const float* h_sigma22);
 void data_set_epsilon(fhs_ctx* ctx, fhs_data* D, const double* h_eps);
 void data_get_norm_res(fhs_ctx* ctx, fhs_data* D, double* h_out);
+void data_check_fb(fhs_ctx* ctx, fhs_data* D);
+void data_inject_fb_timeout(fhs_ctx* ctx, fhs_data* D);
 int data_nblocks(const fhs_data* D);
 int data_channels(const fhs_data* D);
 bool data_square(const fhs_data* D);
@@ -312,6 +314,17 @@ void step(fhs_pdfb* P) {
   ++P->iters;
 }
 
+// A NaN residual sum is how a timed-out exchange of the ellipsoid projection
+// (or non-finite data) shows in the published scalars: raise instead of
+// letting the stopping rule read it as "not converged".
+void check_published(fhs_pdfb* P) {
+  const double* sc = P->h_pub->scal;
+  if (std::isnan(sc[2]) || std::isnan(sc[3])) {
+    data_check_fb(&P->own, P->D);   // raises (and resets the counters) on a time-out
+    throw Error(FHS_ERR_INTERNAL, "pdfb: non-finite data residual (NaN in the data or the iterates)");
+  }
+}
+
 void begin_calls(fhs_pdfb* P) {
   P->own.timing = P->ctx->timing;
   P->side.timing = P->ctx->timing;
@@ -339,11 +352,22 @@ int fhs_pdfb_iterate(fhs_pdfb* P, int n_iter, double* scalars) {
     begin_calls(P);
     for (int it = 0; it < n_iter; ++it) {
       step(P);
+      check_published(P);
       if (scalars)
         for (int k = 0; k < 4; ++k) scalars[4 * it + k] = P->h_pub->scal[k];
     }
     end_calls(P);
   });
+}
+
+int fhs_pdfb_stopping_rule(int rule, int k, double rel_val, double norm_res, double norm_eps, int min_iter,
+                           int max_iter, double rel_var, double fid_tol, double rel_var_low) {
+  // comparisons with NaN are false, as in MATLAB
+  if (!(k >= min_iter)) return 0;
+  if (k >= max_iter) return 1;
+  bool small = rel_val <= rel_var;
+  if (rule == 1) small = small && norm_res <= fid_tol * norm_eps;
+  return (small || rel_val <= rel_var_low) ? 1 : 0;
 }
 
 int fhs_pdfb_run(fhs_pdfb* P, int rule, int k0, int min_iter, int max_iter, double rel_var, double fid_tol,
@@ -360,17 +384,15 @@ int fhs_pdfb_run(fhs_pdfb* P, int rule, int k0, int min_iter, int max_iter, doub
     *converged = 0;
     for (int i = 0; i < n_max; ++i) {
       step(P);
+      check_published(P);
       const double* sc = P->h_pub->scal;
       const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       for (int j = 0; j < 5; ++j) out[6 * i + j] = sc[j];
       out[6 * i + 5] = secs;
       *n_done = i + 1;
-      // the PDFB stopping rule of the solver (NaN compares false, as in MATLAB)
       const double rel = sc[1] > 0.0 ? std::sqrt(sc[0]) / std::sqrt(sc[1]) : std::nan("");
-      const double nres = std::sqrt(sc[2]), neps = std::sqrt(sc[3]);
-      const int k = k0 + i + 1;
-      const bool small = rule == 1 ? (rel <= rel_var && nres <= fid_tol * neps) : rel <= rel_var;
-      if (k >= min_iter && (k >= max_iter || small || rel <= rel_var_low)) {
+      if (fhs_pdfb_stopping_rule(rule, k0 + i + 1, rel, std::sqrt(sc[2]), std::sqrt(sc[3]), min_iter, max_iter,
+                                 rel_var, fid_tol, rel_var_low)) {
         *converged = 1;
         break;
       }
@@ -402,6 +424,11 @@ int fhs_pdfb_truth_error(fhs_pdfb* P, double* err2) {
 int fhs_pdfb_norm_res(fhs_pdfb* P, double* norm_res) {
   if (!P || !norm_res) return FHS_ERR_ARG;
   return guard(P->ctx, [&] { data_get_norm_res(&P->own, P->D, norm_res); });
+}
+
+int fhs_pdfb_inject_fb_timeout(fhs_pdfb* P) {
+  if (!P) return FHS_ERR_ARG;
+  return guard(P->ctx, [&] { data_inject_fb_timeout(&P->own, P->D); });
 }
 
 int fhs_pdfb_stream(fhs_pdfb* P, void** stream) {

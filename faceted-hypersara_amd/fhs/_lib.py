@@ -95,6 +95,9 @@ PROTOTYPES = {
     'fhs_pdfb_run': [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double, c_double, c_int, c_double,
                      c_double, c_double_p, c_int_p, c_int_p],
     'fhs_pdfb_truth_error': [c_void_p, c_double_p],
+    'fhs_pdfb_stopping_rule': [c_int, c_int, c_double, c_double, c_double, c_int, c_int, c_double, c_double,
+                               c_double],
+    'fhs_pdfb_inject_fb_timeout': [c_void_p],
     'fhs_pdfb_stream': [c_void_p, C.POINTER(c_void_p)],
     'fhs_data_forward': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     'fhs_data_adjoint': [c_void_p, c_void_p, c_void_p, c_void_p, c_float_p, c_void_p],

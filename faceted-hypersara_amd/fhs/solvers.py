@@ -410,7 +410,10 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         info['reweights'].append(t)
         return False
 
-    rule = 1 if kind in ('fhs', 'sara') else 0
+    # the PDFB stopping rule: only Faceted HyperSARA gates the relative
+    # variation on the data fidelity (src/fhs/facetHyperSARA.m:877-881);
+    # SARA (src/sara/sara.m:666-670) and HyperSARA (src/hs/hyperSARA.m:682-686) do not
+    rule = 1 if kind == 'fhs' else 0
     if callback is None:
         # whole cycles inside the library
         err2_stop = -1.0
@@ -449,17 +452,7 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
             info['snr'].append(20 * np.log10(nX0 / sqrt(e2)) if e2 > 0 else float('inf'))
         if callback is not None and callback(t, eng, info) is False:
             break
-        k = t - last_step
-        if kind == 'fhs' or kind == 'sara':
-            # src/fhs/facetHyperSARA.m:877-881 (residual check), sara.m the same rule
-            conv = (k >= prm['pdfb_min_iter']) and (
-                k >= prm['pdfb_max_iter'] or
-                (rel_val <= prm['pdfb_rel_var'] and nres <= prm['pdfb_fidelity_tolerance'] * neps) or
-                rel_val <= prm['pdfb_rel_var_low'])
-        else:
-            conv = (k >= prm['pdfb_min_iter']) and (
-                k >= prm['pdfb_max_iter'] or rel_val <= prm['pdfb_rel_var'] or rel_val <= prm['pdfb_rel_var_low'])
-        if conv:
+        if pdfb_converged(rule, t - last_step, rel_val, nres, neps, prm):
             if end_of_cycle(t):
                 break
             continue
@@ -470,6 +463,17 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     info['iterations'] = t
     info['wall'] = time.perf_counter() - t0
     return eng, info
+
+
+def pdfb_converged(rule, k, rel_val, norm_res, norm_eps, prm):
+    """The library's PDFB stopping rule (``fhs_pdfb_stopping_rule``; rule 0:
+    ``src/sara/sara.m:666-670``, ``src/hs/hyperSARA.m:682-686``; rule 1:
+    ``src/fhs/facetHyperSARA.m:877-881``) -- the same code ``fhs_pdfb_run``
+    evaluates per iteration."""
+    return bool(lib().fhs_pdfb_stopping_rule(
+        int(rule), int(k), float(rel_val), float(norm_res), float(norm_eps), int(prm['pdfb_min_iter']),
+        int(prm['pdfb_max_iter']), float(prm['pdfb_rel_var']), float(prm['pdfb_fidelity_tolerance']),
+        float(prm['pdfb_rel_var_low'])))
 
 
 def _reweight(eng, alpha, sig, sig_bar):

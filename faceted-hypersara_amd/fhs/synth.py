@@ -125,6 +125,37 @@ def make_problem(Ny, Nx, L, M_base, nblocks, seed=0, flag_range=(0.1, 0.3), inpu
     return Problem(Ny, Nx, L, X0, us, vs, blocks, nu, seed, input_snr)
 
 
+def host_visibilities(problem):
+    """Noiseless visibilities G_l A x0_l of every channel, computed in FP64 on
+    the host (the min-max coefficients of ``fhs.nufft.Nufft2D`` and numpy's
+    FFT), so a problem's data do not depend on the device that builds it."""
+    from .nufft import Nufft2D
+    P = problem
+    nf = Nufft2D(P.Ny, P.Nx)
+    Kty, Ktx = nf.Kty, nf.Ktx
+    sn = np.outer(nf.ay.s, nf.ax.s)
+    out = []
+    for l in range(P.L):
+        pad = np.zeros((Kty, Ktx), dtype=np.complex128)
+        pad[:P.Ny, :P.Nx] = sn * P.X0[:, :, l]
+        F = np.fft.fft2(pad)
+        a, cy, cx = nf.separable(P.u[l], P.v[l])
+        iy = (a % Kty)[:, None] + np.arange(nf.Ky)[None, :]
+        ix = (a // Kty)[:, None] + np.arange(nf.Kx)[None, :]
+        iy %= Kty
+        ix %= Ktx
+        y = np.zeros(len(a), dtype=np.complex128)
+        for jx in range(nf.Kx):
+            y += cx[:, jx] * np.einsum('mj,mj->m', cy, F[iy, ix[:, jx:jx + 1]])
+        out.append(y)
+    return out
+
+
+def add_host_data(problem):
+    """Noisy whitened data from ``host_visibilities`` (``add_noise``)."""
+    return add_noise(problem, host_visibilities(problem))
+
+
 def add_noise(problem, y0):
     """Noisy whitened data from noiseless visibilities y0 (list per channel).
     Returns per-channel sigma (G must be divided by it)."""

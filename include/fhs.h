@@ -339,12 +339,18 @@ int fhs_pdfb_iterate(fhs_pdfb* p, int n_iter, double* scalars);
 int fhs_pdfb_set_truth(fhs_pdfb* p, const float* X0);
 /* last iteration's sum (X0 - x)^2 (host copy, no GPU call) */
 int fhs_pdfb_truth_error(fhs_pdfb* p, double* err2);
-/* Runs iterations until the PDFB stopping rule of the solver holds for
- * iteration k = k0 + i (src/fhs/facetHyperSARA.m:877-881, rule 1, also
- * src/sara/sara.m; src/hs/hyperSARA.m, rule 0):
- *   k >= min_iter && (k >= max_iter || small || rel_x <= rel_var_low),
- *   small = rel_x <= rel_var (&& norm_res <= fid_tol * norm_eps for rule 1),
- * rel_x = sqrt(sum dx^2 / sum x^2); or until n_max iterations, or until
+/* The PDFB (inner) stopping rule of the solvers, evaluated on the host:
+ *   k >= min_iter && (k >= max_iter || small || rel_val <= rel_var_low),
+ *   small = rel_val <= rel_var, and for rule 1 also norm_res <= fid_tol * norm_eps.
+ * rule 0: src/sara/sara.m:666-670 and src/hs/hyperSARA.m:682-686 (no
+ * data-fidelity term); rule 1: src/fhs/facetHyperSARA.m:877-881.  k counts
+ * iterations since the last reweighting; NaN compares false, as in MATLAB.
+ * Returns 1 when the cycle has converged, else 0. */
+int fhs_pdfb_stopping_rule(int rule, int k, double rel_val, double norm_res, double norm_eps, int min_iter,
+                           int max_iter, double rel_var, double fid_tol, double rel_var_low);
+/* Runs iterations until fhs_pdfb_stopping_rule(rule, k, ...) holds for
+ * iteration k = k0 + i, rel_val = sqrt(sum dx^2 / sum x^2), norm_res and
+ * norm_eps the square roots of the published sums; or until n_max iterations, or until
  * sum (X0 - x)^2 <= err2_stop (err2_stop >= 0; needs fhs_pdfb_set_truth),
  * or after seconds_max seconds of wall-clock (> 0).  The per-iteration loop
  * of the solvers without a host round trip per iteration.
@@ -355,6 +361,12 @@ int fhs_pdfb_truth_error(fhs_pdfb* p, double* err2);
 int fhs_pdfb_run(fhs_pdfb* p, int rule, int k0, int min_iter, int max_iter, double rel_var, double fid_tol,
                  double rel_var_low, int n_max, double err2_stop, double seconds_max, double* out,
                  int* n_done, int* converged);
+/* fhs_pdfb_iterate and fhs_pdfb_run fail (FHS_ERR_INTERNAL) when an
+ * iteration's data residual is NaN: a workgroup exchange of the ellipsoid
+ * projection that timed out (its counters are then reset, so the next call
+ * starts clean), or non-finite data.  Test hook: marks the exchange as
+ * timed out from the host, as a real time-out would. */
+int fhs_pdfb_inject_fb_timeout(fhs_pdfb* p);
 /* last iteration's per-block residual norms ||G_b A x - y_b||: host double[nblocks] */
 int fhs_pdfb_norm_res(fhs_pdfb* p, double* norm_res);
 int fhs_pdfb_stream(fhs_pdfb* p, void** stream);

@@ -1,16 +1,31 @@
-"""Oracle: PDFB solvers (HyperSARA, SARA), float64.
+"""Oracle: PDFB solvers (SARA, HyperSARA, Faceted HyperSARA), float64.
 
 Test infrastructure only (see ``oracle/__init__.py``).
 
-``problem`` is a ``OracleProblem``: per-channel NUFFT operators, block row
+``problem`` is an ``OracleProblem``: per-channel NUFFT operators, block row
 ranges, visibilities ``y``, preconditioning weights ``pU`` and l2 bounds.
+
+One loop, ``pdfb``, restates the three reference solvers; where they differ
+(step sizes, prior steps, the stopping rule) the branch cites the file:line
+it follows.  Left out, as in the reference's default run: noise-level
+adjustment (``adjust_flag_noise`` false, ``imaging/default_parameters.json:70``),
+checkpointing and logging.
+
+How the loop is pinned (``tests/test_oracle_solvers.py``): the stopping and
+reweighting rules by hand-built sequences that drive every branch of every
+rule, checked against the ``.m`` text; the whole loop by the optimality
+conditions of the problem it solves, checked at a converged iterate (primal
+stationarity, the dual variables in the subdifferentials of the priors and
+the normal cone of the l2 ball).  The MATLAB iterates themselves cannot be
+produced here (no MATLAB): parity unpinned at that level.
 """
 import numpy as np
 
 from .data_fidelity import update_dual_data_fidelity
+from .facets import faceted_lowrank_step, facet_singular_values
 from .nufft import NufftOracle
 from .prox import nuclear_prox_dual, soft_l21, soft_l1_dual, update_primal
-from .wavelets import SaraDictionary
+from .wavelets import SaraDictionary, DEFAULT_BASES, DEFAULT_NLEVEL
 
 
 class OracleProblem:
@@ -61,21 +76,17 @@ class OracleProblem:
         P.dr = True
         return P
 
+    def forward(self, x):
+        """Per channel the stacked blocks of G_l A x_l (whitened measurement
+        operator, ``lib/operators/HS_forward_operator_G.m``)."""
+        return [np.concatenate([G @ self.A[l](x[:, :, l]) for G in self.G[l]]) for l in range(self.L)]
 
-def hs_initial_weights(x, sara, alpha, sig, sig_bar):
-    """hs_initialize_dual_lowrankness_serial.m:25-29 and
-    hs_initialize_dual_sparsity_distributed.m:24-35 (and the per-reweight
-    hs_update_weights_*.m, identical formulas)."""
-    Ny, Nx, L = x.shape
-    X = x.reshape((-1, L), order='F')
-    d0 = np.abs(np.linalg.svd(X, compute_uv=False))
-    ub = sig_bar * alpha
-    w0 = ub / (ub + d0)
-    r = np.stack([sara.psit(x[:, :, l]) for l in range(L)], axis=1)
-    d1 = np.sqrt(np.sum(r ** 2, axis=1))
-    up = sig * alpha
-    w1 = up / (up + d1)
-    return w0, w1
+    def adjoint_blocks(self, v, l):
+        """real(At(sum_j G_j' v_j)) of channel l (``HS_adjoint_operator_G.m``)."""
+        g = np.zeros(self.No, dtype=np.complex128)
+        for G, vj in zip(self.G[l], v):
+            g += G.conj().T @ vj
+        return np.real(self.At[l](g))
 
 
 def snr(x0, x):
@@ -83,208 +94,234 @@ def snr(x0, x):
     return 20 * np.log10(np.linalg.norm(x0.ravel()) / np.linalg.norm((x0 - x).ravel()))
 
 
-def hypersara(problem, param, x_init=None, X0=None, max_iter_total=None, callback=None):
-    """PDFB + reweighting of ``src/hs/hyperSARA.m`` (no checkpointing, no noise
-    adjustment: ``adjust_flag_noise`` false as in
-    ``imaging/default_parameters.json:70``).
+# --------------------------------------------------------------- rules --
 
-    param keys: nu2 (per channel), gamma, gamma0, sig, sig_bar, alpha,
-    pdfb_min_iter, pdfb_max_iter, pdfb_rel_var, pdfb_rel_var_low,
-    reweighting_min_iter, reweighting_max_iter, reweighting_rel_var,
-    elipse_proj_max_iter, elipse_proj_min_iter, elipse_proj_eps.
-    Returns (xsol, info).
-    """
+def pdfb_converged(kind, k, rel_val, norm_res, norm_eps, param):
+    """Inner (PDFB) stopping rule at iteration k counted from the last
+    reweighting (``t - redefine_min_task_last_step_iter``).
+
+    'sara' (``src/sara/sara.m:666-670``) and 'hs' (``src/hs/hyperSARA.m:682-686``):
+        k >= min_iter && (k >= max_iter || rel_val <= rel_var || rel_val <= rel_var_low)
+    'fhs' (``src/fhs/facetHyperSARA.m:877-881``): the rel_var branch also needs
+        the data fidelity, norm_res <= fidelity_tolerance * norm_eps.
+    NaN compares false, as in MATLAB."""
+    if not k >= param['pdfb_min_iter']:
+        return False
+    if k >= param['pdfb_max_iter']:
+        return True
+    small = rel_val <= param['pdfb_rel_var']
+    if kind == 'fhs':
+        small = small and norm_res <= param['pdfb_fidelity_tolerance'] * norm_eps
+    return bool(small or rel_val <= param['pdfb_rel_var_low'])
+
+
+def reweighting_converged(rw_count, rel_x_reweighting, param, reweighting_flag=True):
+    """Outer (reweighting) stopping rule, evaluated when the PDFB converged:
+    ``src/sara/sara.m:713-717``, ``src/hs/hyperSARA.m:748-752``,
+    ``src/fhs/facetHyperSARA.m:965-969`` (identical)."""
+    return (not reweighting_flag) or (
+        rw_count >= param['reweighting_min_iter'] and
+        (rw_count >= param['reweighting_max_iter'] or rel_x_reweighting <= param['reweighting_rel_var']))
+
+
+def rel_x_reweighting(xlast, xsol):
+    """||xlast - xsol|| / ||xlast|| (``src/hs/hyperSARA.m:744``; the facet sums of
+    ``src/fhs/facetHyperSARA.m:950-963`` are the same quantity).  MATLAB's
+    x/0 is Inf (or NaN for 0/0), which never passes ``<= rel_var``."""
+    d = np.linalg.norm((xlast - xsol).ravel())
+    n = np.linalg.norm(xlast.ravel())
+    if n > 0:
+        return d / n
+    return np.inf if d > 0 else np.nan
+
+
+# ------------------------------------------------------------- weights --
+
+def sparsity_weights(x, sara, upsilon):
+    """l21 weights upsilon ./ (upsilon + ||Psit(x)||_row):
+    ``src/hs/hs_update_weights_sparsity_distributed.m``,
+    ``src/fhs/fhs_update_weights.m:82-92``; for L = 1 the SARA weights
+    ``upsilon ./ (upsilon + abs(Psit(x)))`` (``src/sara/sara.m:764-769``)."""
+    L = x.shape[2]
+    r = np.stack([sara.psit(x[:, :, l]) for l in range(L)], axis=1)
+    d = np.sqrt(np.sum(r ** 2, axis=1))
+    return upsilon / (upsilon + d)
+
+
+def lowrank_weights(x, upsilon_bar):
+    """``src/hs/hs_update_weights_lowrankness_serial.m``:
+    upsilon_bar ./ (upsilon_bar + svd(reshape(x, [], L)))."""
+    L = x.shape[2]
+    d = np.abs(np.linalg.svd(x.reshape((-1, L), order='F'), compute_uv=False))
+    return upsilon_bar / (upsilon_bar + d)
+
+
+def hs_initial_weights(x, sara, alpha, sig, sig_bar):
+    """``hs_initialize_dual_lowrankness_serial.m:25-29`` and
+    ``hs_initialize_dual_sparsity_distributed.m:24-35`` (the same formulas
+    as the per-reweighting ``hs_update_weights_*.m``): (w0, w1)."""
+    return lowrank_weights(x, sig_bar * alpha), sparsity_weights(x, sara, sig * alpha)
+
+
+# ---------------------------------------------------------------- loop --
+
+def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=None, callback=None):
+    """PDFB + reweighting of one of the reference solvers, kind 'sara'
+    (``src/sara/sara.m:521-840``), 'hs' (``src/hs/hyperSARA.m:565-790``) or
+    'fhs' (``src/fhs/facetHyperSARA.m:701-1140``, one process; ``geom`` the
+    overlapping nuclear-norm facets).
+
+    param keys: nu2 (per channel), gamma, sig, alpha (reweighting_alpha),
+    pdfb_* and reweighting_* of ``imaging/default_parameters.json:5-21``,
+    elipse_proj_*; for 'hs'/'fhs' also gamma0 and sig_bar (per facet for
+    'fhs'); optional 'bases'/'nlevel' (default db1-db8 + Dirac, 4 levels).
+    Returns (xsol, info) with the per-iteration rel_val, norm_res, norm_eps,
+    snr, the reweighting iterations and the final duals and weights."""
     P = problem
     Ny, Nx, L = P.Ny, P.Nx, P.L
     N = Ny * Nx
-    sara = SaraDictionary(Ny, Nx)
-    xsol = np.zeros((Ny, Nx, L)) if x_init is None else np.array(x_init, dtype=np.float64)
+    if kind == 'sara' and L != 1:
+        raise ValueError('sara: one channel')
+    sara = SaraDictionary(Ny, Nx, param.get('bases', DEFAULT_BASES), param.get('nlevel', DEFAULT_NLEVEL))
+    xsol = np.zeros((Ny, Nx, L)) if x_init is None else np.array(x_init, dtype=np.float64).reshape(Ny, Nx, L)
     g = np.zeros_like(xsol)
-    xlast = xsol.copy()
-    w0, w1 = hs_initial_weights(xsol, sara, param['alpha'], param['sig'], param['sig_bar'])
-    if not np.any(xsol):
-        w0 = np.ones(min(N, L)); w1 = np.ones(sara.s)
-    v0 = np.zeros((N, L))
-    v1 = np.zeros((sara.s, L))
-    v2 = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[l]] for l in range(L)]
-    proj = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[l]] for l in range(L)]
-    Fx_old = np.stack([P.A[l](xsol[:, :, l]) for l in range(L)], axis=1)
-    tau = 0.99 / 3
-    sigma00 = tau * 1.0
-    sigma11 = tau * 1.0
-    sigma22 = tau / np.asarray(param['nu2'], dtype=np.float64)
-    beta0 = param['gamma0'] / 1.0
-    beta1 = param['gamma'] / 1.0
-    max_iter = (param['reweighting_max_iter'] + 1) * param['pdfb_max_iter']
-    if max_iter_total is not None:
-        max_iter = min(max_iter, max_iter_total)
-    rw_count = 0
-    last_step = 0
-    info = {'rel_val': [], 'snr': [], 'norm_res': [], 'norm_eps': [], 'reweights': []}
-    t = 0
-    for t in range(1, max_iter + 1):
-        prev = xsol
-        xsol, xhat, rel_x, norm_x = update_primal(xsol, g)
-        # nuclear prox (hs_update_dual_lowrankness_serial.m:5-10)
-        Z = v0 + xhat.reshape((N, L), order='F')
-        v0 = nuclear_prox_dual(Z, beta0 * w0)
-        g0 = sigma00 * v0.reshape((Ny, Nx, L), order='F')
-        # l21 prox (hs_update_dual_sparsity_distributed.m:4-17)
-        r1 = v1 + np.stack([sara.psit(xhat[:, :, l]) for l in range(L)], axis=1)
-        v1 = soft_l21(r1, beta1 * w1)
-        g1 = np.stack([sigma11 * sara.psi(v1[:, l]) for l in range(L)], axis=2)
-        # data fidelity
-        v2, g2, Fx_old, proj, norm_res, gnr, neps = update_dual_data_fidelity(
-            v2, P.y, xsol, Fx_old, proj, P.A, P.At, P.G, P.W, P.pU, P.epsilon,
-            param['elipse_proj_max_iter'], param['elipse_proj_min_iter'],
-            param['elipse_proj_eps'], sigma22, flag_dimensionality_reduction=P.dr, Lambda=P.Lambda)
-        g = g0 + g1 + g2
-        nx = np.linalg.norm(xsol.ravel())
-        rel_val = np.linalg.norm((prev - xsol).ravel()) / nx if nx > 0 else np.nan
-        info['rel_val'].append(rel_val)
-        info['norm_res'].append(np.sqrt(gnr))
-        info['norm_eps'].append(np.sqrt(neps))
-        if X0 is not None:
-            info['snr'].append(snr(X0, xsol))
-        if callback is not None:
-            callback(t, xsol, info)
-        k = t - last_step
-        pdfb_converged = (k >= param['pdfb_min_iter']) and (
-            k >= param['pdfb_max_iter'] or rel_val <= param['pdfb_rel_var']
-            or rel_val <= param['pdfb_rel_var_low'])
-        if pdfb_converged:
-            rel_rw = np.linalg.norm((xlast - xsol).ravel()) / np.linalg.norm(xlast.ravel()) \
-                if np.any(xlast) else np.inf
-            xlast = xsol.copy()
-            converged = (rw_count >= param['reweighting_min_iter'] and
-                         (rw_count >= param['reweighting_max_iter'] or
-                          rel_rw <= param['reweighting_rel_var']))
-            if converged:
-                break
-            w0, w1 = hs_initial_weights(xsol, sara, param['alpha'], param['sig'], param['sig_bar'])
-            rw_count += 1
-            last_step = t
-            info['reweights'].append(t)
-    info['iterations'] = t
-    info['v0'], info['v1'], info['v2'], info['g'] = v0, v1, v2, g
-    return xsol, info
-
-
-def sara_solver(problem, param, x_init=None, X0=None, max_iter_total=None):
-    """Single-channel PDFB of ``src/sara/sara.m:521-633`` (FP64, no reweighting
-    when ``reweighting_max_iter`` is 0): per-basis l1 dual steps
-    (``run_par_waverec``, ``src/sara/sara.m:921-932``) and the data term with
-    step ``tau = 0.99/(sigma1 nu1 + sigma2 nu2)`` (``src/sara/sara.m:505``)."""
-    P = problem
-    Ny, Nx = P.Ny, P.Nx
-    sara = SaraDictionary(Ny, Nx)
-    xsol = np.zeros((Ny, Nx, 1)) if x_init is None else np.array(x_init, dtype=np.float64).reshape(Ny, Nx, 1)
-    g = np.zeros_like(xsol)
-    v1 = np.zeros(sara.s)
-    w1 = np.ones(sara.s)
-    v2 = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[0]]]
-    proj = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[0]]]
-    Fx_old = P.A[0](xsol[:, :, 0])[:, None]
-    nu2 = float(np.atleast_1d(param['nu2'])[0])
-    sigma1, sigma2 = 1.0, 1.0 / nu2
-    tau = 0.99 / (sigma1 * 1.0 + sigma2 * nu2)
-    sigma11, sigma22 = tau * sigma1, tau * sigma2
+    alpha = param.get('alpha', param.get('reweighting_alpha', 1.0))
+    nu2 = np.broadcast_to(np.asarray(param['nu2'], dtype=np.float64), (L,))
+    # step sizes and thresholds
+    if kind == 'sara':
+        # src/sara/sara.m:500-510: sigma1 = 1/nu1, sigma2 = 1/nu2, tau = 0.99/(sigma1 nu1 + sigma2 nu2)
+        sigma1, sigma2 = 1.0, 1.0 / nu2
+        tau = 0.99 / (sigma1 * 1.0 + sigma2[0] * nu2[0])
+        sigma00 = None
+    else:
+        # src/hs/hyperSARA.m:471-486, src/fhs/facetHyperSARA.m:582-603 (nu0 = nu1 = 1)
+        sigma0, sigma1, sigma2 = 1.0, 1.0, 1.0 / nu2
+        tau = 0.99 / 3
+        sigma00 = tau * sigma0
+    sigma11 = tau * sigma1
+    sigma22 = tau * sigma2
     beta1 = param['gamma'] / sigma1
-    n_iter = param['pdfb_max_iter'] if max_iter_total is None else max_iter_total
-    info = {'rel_val': [], 'snr': []}
-    for t in range(1, n_iter + 1):
-        prev = xsol
-        xsol, xhat, _, _ = update_primal(xsol, g)
-        r1 = v1 + sara.psit(xhat[:, :, 0])
-        v1 = soft_l1_dual(r1, beta1 * w1)
-        g1 = sara.psi(v1)
-        v2, Ftx, Fx_old, proj, _, _, _ = update_dual_data_fidelity(
-            v2, P.y, xsol, Fx_old, proj, P.A, P.At, P.G, P.W, P.pU, P.epsilon,
-            param['elipse_proj_max_iter'], param['elipse_proj_min_iter'],
-            param['elipse_proj_eps'], [1.0])
-        g = sigma11 * g1[:, :, None] + sigma22 * Ftx
-        nx = np.linalg.norm(xsol.ravel())
-        info['rel_val'].append(np.linalg.norm((prev - xsol).ravel()) / nx if nx > 0 else np.nan)
-        if X0 is not None:
-            info['snr'].append(snr(X0.reshape(Ny, Nx, 1), xsol))
-    return xsol[:, :, 0], info
+    sig = param['sig']
+    Q = 0
+    if kind == 'hs':
+        beta0 = np.atleast_1d(np.asarray(param['gamma0'], dtype=np.float64))[:1] / sigma0
+        sig_bar = np.atleast_1d(np.asarray(param['sig_bar'], dtype=np.float64))[:1]
+    elif kind == 'fhs':
+        Q = len(geom['oy'])
+        beta0 = np.asarray(param['gamma0'], dtype=np.float64).reshape(Q) / sigma0
+        sig_bar = np.asarray(param['sig_bar'], dtype=np.float64).reshape(Q)
 
+    def weights(x):
+        w1 = sparsity_weights(x, sara, sig * alpha)
+        if kind == 'hs':
+            w0 = lowrank_weights(x, sig_bar[0] * alpha)[None, :]
+        elif kind == 'fhs':
+            w0 = (sig_bar[:, None] * alpha) / (sig_bar[:, None] * alpha + facet_singular_values(x, geom))
+        else:
+            w0 = None
+        return w0, w1
 
-def facet_hypersara(problem, param, geom, x_init=None, X0=None, max_iter_total=None):
-    """PDFB of ``src/fhs/facetHyperSARA.m:701-889`` on one process: faceted
-    weighted nuclear norm (overlapping facets ``geom`` with apodisation
-    windows, per-facet beta0_q = gamma0_q / sigma0, ``:598-601``) and the
-    SARA l21 term whose facet coefficient sets tile the global transform, so
-    the global l21 step is the union of the facet steps
-    (``src/fhs/fhs_update_dual_sparsity.m:67-93``).  Stopping rule with the
-    data-fidelity check (``:877-881``).  param as ``hypersara`` with
-    ``sig_bar`` per facet."""
-    from .facets import faceted_lowrank_step, facet_singular_values
-    P = problem
-    Ny, Nx, L = P.Ny, P.Nx, P.L
-    Q = len(geom['oy'])
-    sara = SaraDictionary(Ny, Nx)
-    xsol = np.zeros((Ny, Nx, L)) if x_init is None else np.array(x_init, dtype=np.float64)
-    g = np.zeros_like(xsol)
-    sig_bar = np.asarray(param['sig_bar'], dtype=np.float64).reshape(Q)
-    w0 = np.ones((Q, L))
-    w1 = np.ones(sara.s)
-    if np.any(xsol):
-        ub = sig_bar[:, None] * param['alpha']
-        w0 = ub / (ub + facet_singular_values(xsol, geom))
-        _, w1 = hs_initial_weights(xsol, sara, param['alpha'], param['sig'], 1.0)
-    v0s = [np.zeros((geom['py'][q] * geom['px'][q], L)) for q in range(Q)]
+    # initial weights from x (all ones at x = 0): src/sara/sara.m:355-363,
+    # hs_initialize_*, src/fhs/fhs_initialize_dual_and_weights.m:88-112
+    w0, w1 = weights(xsol)
+    if kind == 'hs':
+        v0 = [np.zeros((N, L))]
+    elif kind == 'fhs':
+        v0 = [np.zeros((geom['py'][q] * geom['px'][q], L)) for q in range(Q)]
+    else:
+        v0 = None
     v1 = np.zeros((sara.s, L))
     v2 = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[l]] for l in range(L)]
     proj = [[np.zeros(len(b), dtype=np.complex128) for b in P.y[l]] for l in range(L)]
     Fx_old = np.stack([P.A[l](xsol[:, :, l]) for l in range(L)], axis=1)
-    tau = 0.99 / 3
-    sigma00, sigma11 = tau, tau
-    sigma22 = tau / np.asarray(param['nu2'], dtype=np.float64)
-    beta0 = np.asarray(param['gamma0'], dtype=np.float64).reshape(Q)
-    beta1 = param['gamma']
-    max_iter = (param['reweighting_max_iter'] + 1) * param['pdfb_max_iter']
+    max_iter = (param['reweighting_max_iter'] + 1) * param['pdfb_max_iter']   # sara.m:449
     if max_iter_total is not None:
         max_iter = min(max_iter, max_iter_total)
-    info = {'rel_val': [], 'snr': [], 'norm_res': [], 'norm_eps': [], 'reweights': []}
     rw_count, last_step = 0, 0
     xlast = xsol.copy()
+    info = {'rel_val': [], 'snr': [], 'norm_res': [], 'norm_eps': [], 'reweights': [], 'converged': False,
+            'tau': tau, 'sigma00': sigma00, 'sigma11': sigma11, 'sigma22': sigma22, 'beta1': beta1}
+    g0 = g1 = g2 = None
     t = 0
     for t in range(1, max_iter + 1):
         prev = xsol
-        xsol, xhat, _, _ = update_primal(xsol, g)
-        v0s, g0 = faceted_lowrank_step(v0s, xhat, geom, beta0[:, None] * w0, 1.0, sigma00)
+        # src/fhs/fhs_update_primal.m:33-38 (sara.m:529-531, hyperSARA.m:574-576)
+        xsol, xhat, rel_x, norm_x = update_primal(xsol, g)
+        # low-rank prior
+        if kind == 'hs':
+            # src/hs/hs_update_dual_lowrankness_serial.m:5-10
+            Z = v0[0] + xhat.reshape((N, L), order='F')
+            v0 = [nuclear_prox_dual(Z, beta0[0] * w0[0])]
+            g0 = sigma00 * v0[0].reshape((Ny, Nx, L), order='F')
+        elif kind == 'fhs':
+            # src/fhs/fhs_update_dual_lowrankness.m:40-45, overlap-add :746-754
+            v0, g0 = faceted_lowrank_step(v0, xhat, geom, beta0[:, None] * w0, 1.0, sigma00)
+        # sparsity prior
         r1 = v1 + np.stack([sara.psit(xhat[:, :, l]) for l in range(L)], axis=1)
-        v1 = soft_l21(r1, beta1 * w1)
+        if kind == 'sara':
+            # src/sara/sara.m:921-926 (per-basis l1 dual step)
+            v1 = soft_l1_dual(r1[:, 0], beta1 * w1)[:, None]
+        else:
+            # src/hs/hs_update_dual_sparsity_distributed.m:4-17,
+            # src/fhs/fhs_update_dual_sparsity.m:83-93
+            v1 = soft_l21(r1, beta1 * w1)
         g1 = np.stack([sigma11 * sara.psi(v1[:, l]) for l in range(L)], axis=2)
+        # data fidelity: src/update_dual_data_fidelity.m (Ftx already times sigma22)
         v2, g2, Fx_old, proj, norm_res, gnr, neps = update_dual_data_fidelity(
             v2, P.y, xsol, Fx_old, proj, P.A, P.At, P.G, P.W, P.pU, P.epsilon,
-            param['elipse_proj_max_iter'], param['elipse_proj_min_iter'], param['elipse_proj_eps'], sigma22)
-        g = g0 + g1 + g2
-        nx = np.linalg.norm(xsol.ravel())
-        rel_val = np.linalg.norm((prev - xsol).ravel()) / nx if nx > 0 else np.nan
+            param['elipse_proj_max_iter'], param['elipse_proj_min_iter'], param['elipse_proj_eps'],
+            sigma22, flag_dimensionality_reduction=P.dr, Lambda=P.Lambda)
+        # src/sara/sara.m:615 g = sigma11 g1 + sigma22 Ftx; hyperSARA.m:585-613 g = g0 + g1 + g2
+        g = g1 + g2 if g0 is None else g0 + g1 + g2
+        # relative variation: sara.m:535, hyperSARA.m:620-622, facetHyperSARA.m:793
+        rel_val = np.sqrt(rel_x) / np.sqrt(norm_x) if norm_x > 0 else np.nan
+        nres, neps_ = np.sqrt(gnr), np.sqrt(neps)
         info['rel_val'].append(rel_val)
-        info['norm_res'].append(np.sqrt(gnr))
-        info['norm_eps'].append(np.sqrt(neps))
+        info['norm_res'].append(nres)
+        info['norm_eps'].append(neps_)
         if X0 is not None:
-            info['snr'].append(snr(X0, xsol))
-        k = t - last_step
-        conv = (k >= param['pdfb_min_iter']) and (
-            k >= param['pdfb_max_iter'] or
-            (rel_val <= param['pdfb_rel_var'] and np.sqrt(gnr) <= param['pdfb_fidelity_tolerance'] * np.sqrt(neps))
-            or rel_val <= param['pdfb_rel_var_low'])
-        if conv:
-            rel_rw = np.linalg.norm((xlast - xsol).ravel()) / np.linalg.norm(xlast.ravel()) \
-                if np.any(xlast) else np.inf
+            info['snr'].append(snr(X0.reshape(xsol.shape), xsol))
+        if callback is not None:
+            callback(t, xsol, info)
+        if pdfb_converged(kind, t - last_step, rel_val, nres, neps_, param):
+            rel_rw = rel_x_reweighting(xlast, xsol)
             xlast = xsol.copy()
-            if (rw_count >= param['reweighting_min_iter'] and
-                    (rw_count >= param['reweighting_max_iter'] or rel_rw <= param['reweighting_rel_var'])):
+            if reweighting_converged(rw_count, rel_rw, param, param.get('reweighting_flag', True)):
+                info['converged'] = True
                 break
-            ub = sig_bar[:, None] * param['alpha']
-            w0 = ub / (ub + facet_singular_values(xsol, geom))
-            _, w1 = hs_initial_weights(xsol, sara, param['alpha'], param['sig'], 1.0)
+            # reweighting: sara.m:764-769, hyperSARA.m (hs_update_weights_*),
+            # facetHyperSARA.m:990-1000 (fhs_update_weights)
+            w0, w1 = weights(xsol)
             rw_count += 1
             last_step = t
             info['reweights'].append(t)
     info['iterations'] = t
+    info['v0'], info['v1'], info['v2'], info['proj'] = v0, v1, v2, proj
+    info['g'], info['g0'], info['g1'], info['g2'] = g, g0, g1, g2
+    info['w0'], info['w1'] = w0, w1
+    info['beta0'] = None if kind == 'sara' else beta0
     return xsol, info
+
+
+def hypersara(problem, param, x_init=None, X0=None, max_iter_total=None, callback=None):
+    """HyperSARA (``src/hs/hyperSARA.m``): ``pdfb('hs', ...)``."""
+    return pdfb('hs', problem, param, x_init=x_init, X0=X0, max_iter_total=max_iter_total, callback=callback)
+
+
+def sara_solver(problem, param, x_init=None, X0=None, max_iter_total=None, callback=None):
+    """SARA (``src/sara/sara.m``): ``pdfb('sara', ...)``; returns (x (Ny, Nx), info).
+    param needs 'sig' (the reweighting noise level) besides 'gamma'."""
+    p = dict(param)
+    p.setdefault('sig', p['gamma'])
+    x, info = pdfb('sara', problem, p, x_init=x_init, X0=X0, max_iter_total=max_iter_total, callback=callback)
+    return x[:, :, 0], info
+
+
+def facet_hypersara(problem, param, geom, x_init=None, X0=None, max_iter_total=None, callback=None):
+    """Faceted HyperSARA (``src/fhs/facetHyperSARA.m``) on one process:
+    ``pdfb('fhs', ...)`` with the overlapping facets ``geom``.  The SARA l21
+    term's facet coefficient sets tile the global transform, so the global
+    l21 step is the union of the facet steps
+    (``src/fhs/fhs_update_dual_sparsity.m:67-93``)."""
+    return pdfb('fhs', problem, param, geom=geom, x_init=x_init, X0=X0, max_iter_total=max_iter_total,
+                callback=callback)
