@@ -41,11 +41,13 @@ def _is_tril(G):
 def update_dual_data_fidelity(v2, y, x, Fx_old, proj, A, At, G, W, pU, epsilon,
                               elipse_proj_max_iter, elipse_proj_min_iter,
                               elipse_proj_eps, sigma22,
-                              flag_dimensionality_reduction=False, Lambda=None):
+                              flag_dimensionality_reduction=False, Lambda=None, mapper=map):
     """``src/update_dual_data_fidelity.m:78-147`` (both branches).
 
     ``A``/``At``: per-channel lists of callables (the reference passes one
     handle; channels of a synthetic cube have their own operators).
+    ``mapper``: how the independent channels are visited (``map``, or a
+    thread pool's map; the arithmetic of each channel is the same).
     Returns (v2, Ftx, Fx_old, proj, norm_res, global_norm_res, norm_epsilon)
     where ``global_norm_res`` and ``norm_epsilon`` are sums of squares.
     """
@@ -54,10 +56,8 @@ def update_dual_data_fidelity(v2, y, x, Fx_old, proj, A, At, G, W, pU, epsilon,
     v2 = [list(b) for b in v2]
     proj = [list(b) for b in proj]
     Fx_old = Fx_old.copy()
-    norm_res = []
-    global_norm_res = 0.0
-    norm_epsilon = 0.0
-    for i in range(n_channels):
+
+    def channel(i):
         Fx = A[i](x[:, :, i])
         g2 = np.zeros(Fx.shape, dtype=np.complex128)
         nr = []
@@ -91,11 +91,18 @@ def update_dual_data_fidelity(v2, y, x, Fx_old, proj, A, At, G, W, pU, epsilon,
                 v2[i][j] = v2[i][j] + pU[i][j] * (r2 - proj[i][j])
                 g2[Wij] += Gij.conj().T @ v2[i][j]
                 nr.append(float(np.linalg.norm(Gij @ Fx[Wij] - y[i][j])))
-            global_norm_res += nr[-1] ** 2
-            norm_epsilon += epsilon[i][j] ** 2
-        norm_res.append(nr)
+        return Fx, sigma22[i] * np.real(At[i](g2)), nr
+
+    norm_res = []
+    global_norm_res = 0.0
+    norm_epsilon = 0.0
+    for i, (Fx, ftx, nr) in enumerate(mapper(channel, range(n_channels))):
         Fx_old[:, i] = Fx
-        Ftx[:, :, i] = sigma22[i] * np.real(At[i](g2))
+        Ftx[:, :, i] = ftx
+        norm_res.append(nr)
+        for j, r in enumerate(nr):        # sums in channel, block order
+            global_norm_res += r ** 2
+            norm_epsilon += epsilon[i][j] ** 2
     return v2, Ftx, Fx_old, proj, norm_res, global_norm_res, norm_epsilon
 
 

@@ -137,13 +137,16 @@ def rel_x_reweighting(xlast, xsol):
 
 # ------------------------------------------------------------- weights --
 
-def sparsity_weights(x, sara, upsilon):
+def sparsity_weights(x, sara, upsilon, psit_all=None):
     """l21 weights upsilon ./ (upsilon + ||Psit(x)||_row):
     ``src/hs/hs_update_weights_sparsity_distributed.m``,
     ``src/fhs/fhs_update_weights.m:82-92``; for L = 1 the SARA weights
     ``upsilon ./ (upsilon + abs(Psit(x)))`` (``src/sara/sara.m:764-769``)."""
     L = x.shape[2]
-    r = np.stack([sara.psit(x[:, :, l]) for l in range(L)], axis=1)
+    if psit_all is not None:
+        r = psit_all(x)
+    else:
+        r = np.stack([sara.psit(x[:, :, l]) for l in range(L)], axis=1)
     d = np.sqrt(np.sum(r ** 2, axis=1))
     return upsilon / (upsilon + d)
 
@@ -165,7 +168,8 @@ def hs_initial_weights(x, sara, alpha, sig, sig_bar):
 
 # ---------------------------------------------------------------- loop --
 
-def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=None, callback=None):
+def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=None, callback=None,
+         threads=1):
     """PDFB + reweighting of one of the reference solvers, kind 'sara'
     (``src/sara/sara.m:521-840``), 'hs' (``src/hs/hyperSARA.m:565-790``) or
     'fhs' (``src/fhs/facetHyperSARA.m:701-1140``, one process; ``geom`` the
@@ -175,8 +179,18 @@ def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=N
     pdfb_* and reweighting_* of ``imaging/default_parameters.json:5-21``,
     elipse_proj_*; for 'hs'/'fhs' also gamma0 and sig_bar (per facet for
     'fhs'); optional 'bases'/'nlevel' (default db1-db8 + Dirac, 4 levels).
+    ``threads`` > 1 visits the independent channels (wavelet transforms, the
+    data step) from a thread pool; the arithmetic is unchanged.
     Returns (xsol, info) with the per-iteration rel_val, norm_res, norm_eps,
     snr, the reweighting iterations and the final duals and weights."""
+    if threads > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(threads) as ex:
+            return _pdfb(kind, problem, param, geom, x_init, X0, max_iter_total, callback, ex.map)
+    return _pdfb(kind, problem, param, geom, x_init, X0, max_iter_total, callback, map)
+
+
+def _pdfb(kind, problem, param, geom, x_init, X0, max_iter_total, callback, mapper):
     P = problem
     Ny, Nx, L = P.Ny, P.Nx, P.L
     N = Ny * Nx
@@ -211,8 +225,14 @@ def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=N
         beta0 = np.asarray(param['gamma0'], dtype=np.float64).reshape(Q) / sigma0
         sig_bar = np.asarray(param['sig_bar'], dtype=np.float64).reshape(Q)
 
+    def psit_all(x):
+        return np.stack(list(mapper(lambda l: sara.psit(x[:, :, l]), range(L))), axis=1)
+
+    def psi_all(v, scale):
+        return np.stack(list(mapper(lambda l: scale * sara.psi(v[:, l]), range(L))), axis=2)
+
     def weights(x):
-        w1 = sparsity_weights(x, sara, sig * alpha)
+        w1 = sparsity_weights(x, sara, sig * alpha, psit_all)
         if kind == 'hs':
             w0 = lowrank_weights(x, sig_bar[0] * alpha)[None, :]
         elif kind == 'fhs':
@@ -257,7 +277,7 @@ def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=N
             # src/fhs/fhs_update_dual_lowrankness.m:40-45, overlap-add :746-754
             v0, g0 = faceted_lowrank_step(v0, xhat, geom, beta0[:, None] * w0, 1.0, sigma00)
         # sparsity prior
-        r1 = v1 + np.stack([sara.psit(xhat[:, :, l]) for l in range(L)], axis=1)
+        r1 = v1 + psit_all(xhat)
         if kind == 'sara':
             # src/sara/sara.m:921-926 (per-basis l1 dual step)
             v1 = soft_l1_dual(r1[:, 0], beta1 * w1)[:, None]
@@ -265,12 +285,12 @@ def pdfb(kind, problem, param, geom=None, x_init=None, X0=None, max_iter_total=N
             # src/hs/hs_update_dual_sparsity_distributed.m:4-17,
             # src/fhs/fhs_update_dual_sparsity.m:83-93
             v1 = soft_l21(r1, beta1 * w1)
-        g1 = np.stack([sigma11 * sara.psi(v1[:, l]) for l in range(L)], axis=2)
+        g1 = psi_all(v1, sigma11)
         # data fidelity: src/update_dual_data_fidelity.m (Ftx already times sigma22)
         v2, g2, Fx_old, proj, norm_res, gnr, neps = update_dual_data_fidelity(
             v2, P.y, xsol, Fx_old, proj, P.A, P.At, P.G, P.W, P.pU, P.epsilon,
             param['elipse_proj_max_iter'], param['elipse_proj_min_iter'], param['elipse_proj_eps'],
-            sigma22, flag_dimensionality_reduction=P.dr, Lambda=P.Lambda)
+            sigma22, flag_dimensionality_reduction=P.dr, Lambda=P.Lambda, mapper=mapper)
         # src/sara/sara.m:615 g = sigma11 g1 + sigma22 Ftx; hyperSARA.m:585-613 g = g0 + g1 + g2
         g = g1 + g2 if g0 is None else g0 + g1 + g2
         # relative variation: sara.m:535, hyperSARA.m:620-622, facetHyperSARA.m:793
