@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "kernels.hpp"
 
 hipEvent_t fhs_ctx::take_event() {
   if (!event_pool.empty()) {
@@ -51,6 +52,7 @@ int fhs_ctx_create(int device, fhs_ctx** out) {
     FHS_CHECK(device >= 0 && device < n, "fhs_ctx_create: device out of range");
     FHS_HIP(hipSetDevice(device));
     ctx->device = device;
+    fhs::noalloc_register_device(device);
   });
   if (st != FHS_OK) {
     delete ctx;

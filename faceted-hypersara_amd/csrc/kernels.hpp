@@ -19,4 +19,8 @@ void nufft_forward(fhs_ctx* ctx, const fhs_nufft* p, const float* x, int L, floa
 void nufft_adjoint(fhs_ctx* ctx, const fhs_nufft* p, const float2* g, int L, const float* scale_dev,
                    float* out);
 
+// Registers this device's address of the fhs_proj_l2ball_noalloc scratch
+// (with `device` current; once per device, from fhs_ctx_create).
+void noalloc_register_device(int device);
+
 }  // namespace fhs

@@ -3,6 +3,8 @@
 // Reductions are two-level and deterministic: a fixed grid of RED_BLOCKS
 // workgroups writes one partial per workgroup, a single workgroup sums them
 // in a fixed order.
+#include <mutex>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -107,7 +109,32 @@ unsigned red_grid(int64_t n) {
   return (unsigned)(b < RED_BLOCKS ? (b < 1 ? 1 : b) : RED_BLOCKS);
 }
 
+constexpr int NOALLOC_PARTIALS = RED_BLOCKS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};   // per device: the address of `scratch`
+
 }  // namespace
+
+// Scratch of fhs_proj_l2ball_noalloc: the FP64 partial sums of its
+// reduction, one per workgroup.  The reduction grid is capped at RED_BLOCKS
+// workgroups whatever n is, so this fixed array (16 KiB per device, held
+// until the process ends) serves every size.  Host code never passes the
+// array's name to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_register_device).
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+// Called by fhs_ctx_create with `device` current: looks up this device's copy
+// of `scratch` once and keeps its address.
+void noalloc_register_device(int device) {
+  FHS_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, "fhs_ctx_create: device index too large");
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (noalloc_ptr[device]) return;
+  void* ptr = nullptr;
+  FHS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  FHS_CHECK(ptr != nullptr, "fhs_ctx_create: no device address for the proj_l2ball scratch");
+  noalloc_ptr[device] = (double*)ptr;
+}
 
 void primal_update(fhs_ctx* ctx, float* x, const float* ga, const float* gb, const float* gc,
                    float* xhat, int64_t n, double* d_out2, const float* truth, double* d_truth) {
@@ -136,6 +163,27 @@ int fhs_proj_l2ball(fhs_ctx* ctx, const void* x, double eps, const void* y, int6
     const unsigned G = red_grid(n);
     double* part = (double*)ctx->ws.get("proj_partial", G * sizeof(double));
     Scope sc(ctx, "proj_l2ball");
+    k_sqdiff_partial<<<G, RED_THREADS, 0, ctx->stream>>>((const float2*)x, (const float2*)y, n, part);
+    check_launch();
+    k_proj_apply<<<G, RED_THREADS, 0, ctx->stream>>>((const float2*)x, (const float2*)y, n, eps, part,
+                                                     (int)G, (float2*)p);
+    check_launch();
+  });
+}
+
+int fhs_proj_l2ball_noalloc(fhs_ctx* ctx, const void* x, double eps, const void* y, int64_t n, void* p) {
+  if (!ctx) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(x && y && p && n >= 0, "proj_l2ball_noalloc: bad argument");
+    FHS_CHECK(eps >= 0.0, "proj_l2ball_noalloc: eps must be >= 0");
+    if (n == 0) return;
+    FHS_CHECK(ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES && noalloc_ptr[ctx->device],
+              "proj_l2ball_noalloc: scratch not registered for this device (context not created by fhs_ctx_create)");
+    double* part = noalloc_ptr[ctx->device];
+    const unsigned G = red_grid(n);
+    FHS_CHECK(G <= (unsigned)NOALLOC_PARTIALS, "proj_l2ball_noalloc: reduction grid exceeds the scratch capacity");
+    // every partial the second kernel reads is written by the first one on
+    // the same stream, so nothing needs clearing between calls
     k_sqdiff_partial<<<G, RED_THREADS, 0, ctx->stream>>>((const float2*)x, (const float2*)y, n, part);
     check_launch();
     k_proj_apply<<<G, RED_THREADS, 0, ctx->stream>>>((const float2*)x, (const float2*)y, n, eps, part,

@@ -59,6 +59,7 @@ PROTOTYPES = {
     'fhs_apply_scaled_fourier_transform': [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_apply_scaled_fourier_adjoint': [c_void_p, c_void_p, c_void_p, c_int, c_float_p, c_void_p],
     'fhs_proj_l2ball': [c_void_p, c_void_p, c_double, c_void_p, c_i64, c_void_p],
+    'fhs_proj_l2ball_noalloc': [c_void_p, c_void_p, c_double, c_void_p, c_i64, c_void_p],
     'fhs_data_create': [c_void_p, c_void_p, c_int, C.POINTER(c_void_p), c_int, c_i64_p, c_int_p,
                         C.POINTER(c_void_p)],
     'fhs_data_destroy': [c_void_p],

@@ -157,6 +157,18 @@ int fhs_apply_scaled_fourier_adjoint(fhs_ctx* ctx, const fhs_nufft* p,
 int fhs_proj_l2ball(fhs_ctx* ctx, const void* x, double eps, const void* y,
                     int64_t n, void* p);
 
+/* fhs_proj_l2ball without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same kernels
+ * and results as fhs_proj_l2ball; its reduction partials live in a fixed
+ * 16 KiB array per device (2048 FP64 partials, enough for every n: the
+ * reduction grid is capped at 2048 workgroups), registered by
+ * fhs_ctx_create.  The context must come from fhs_ctx_create.  Calls that
+ * overlap on one GPU share that array and race: issue them in order (one
+ * stream, or streams ordered by events). */
+int fhs_proj_l2ball_noalloc(fhs_ctx* ctx, const void* x, double eps, const void* y, int64_t n,
+                            void* p);
+
 /* [epsilon, t_block] = update_epsilon(epsilon, t, t_block, norm_res,
  *   adapt_eps_tol_in, adapt_eps_tol_out, adapt_eps_steps,
  *   adapt_eps_change_percentage)   src/update_epsilon.m:1-73
