@@ -13,11 +13,16 @@ dual steps over all 16 channels, as in src/hs/hyperSARA.m:565-617) with its
 stopping-rule scalars read back.  Prints one JSON line.
 
 Besides iterations/s the line carries: the dominant kernel against the HBM
-roofline (HIP-event timing on the engine stream, algorithmic bytes), the
-wall-clock and iterations to reach the converged SNR minus 0.1 ... 3 dB for
-the plain zero start and for the learned warm start (``solve``; the SNR is
-evaluated on the GPU every iteration), and one FP64 oracle iteration on the
-host as the CPU baseline.
+roofline (HIP-event timing on the engine stream, algorithmic bytes), and
+``solve``: the wall-clock and iterations to reach the reference SNR minus
+0.1 ... 3 dB for three GPU forms -- plain (the drop-in form: the
+reference-named entry points called one after the other every iteration,
+zero start), batched (the PDFB engine, one hipGraph replay per iteration)
+and warm_start (the engine from the learned start) -- with each form's
+speed-up over plain.  The reference SNR is the FP64 oracle's converged SNR
+on the same data (tests/golden/hs512x16_reference.npz) when the fixture
+exists for the workload.  The CPU baseline is the FP64 oracle on the host
+cores (10 iterations of the same problem).
 """
 import argparse
 import json
@@ -220,6 +225,16 @@ def main():
         prob = synth.make_problem(args.size, args.size, 1, args.vis, 2, seed=args.seed)
     else:
         prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed)
+    # The headline workload's reference: the FP64 oracle's solve of the same
+    # problem run to convergence (tests/golden/make_hs512_reference.py), on
+    # data computed in FP64 on the host so both solve identical problems.
+    ref_fx = None
+    if args.config == 'hs512':
+        synth.add_host_data(prob)
+        fx_path = os.path.join(ROOT, 'tests', 'golden', 'hs512x16_reference.npz')
+        if (args.size, args.channels, args.vis, args.seed) == (512, 16, 200000, 0) and os.path.exists(fx_path):
+            with np.load(fx_path) as z:
+                ref_fx = {k: z[k] for k in z.files}
     kind = 'fhs' if facets is not None else ('sara' if args.config == 'sara256' else 'hs')
     dr = args.config == 'hs512dr'
     if dr:
@@ -231,6 +246,12 @@ def main():
         meas = solvers.Measurement(ctx, prob, precond=True)
         son = meas.op_norm(False)
         nu2 = meas.op_norm(True)
+    opnorm_gap = None
+    if ref_fx is not None:
+        # the oracle's FP64 operator norms, so the step sizes and thresholds
+        # are the oracle run's; the GPU power iteration's distance to them
+        opnorm_gap = float(max(np.max(np.abs(son / ref_fx['son'] - 1)), np.max(np.abs(nu2 / ref_fx['nu2'] - 1))))
+        son, nu2 = ref_fx['son'], ref_fx['nu2']
     setup_s = time.perf_counter() - t_setup
     log(f'setup {setup_s:.1f} s')
     param = {'squared_operator_norm': son, 'nu2': nu2, 'reweighting_max_iter': 3}
@@ -281,7 +302,7 @@ def main():
                    'GBps': (alg[k] / (rep[k][0] / rep[k][1] * 1e-3) / 1e9) if k in alg else None}
                for k in rep}
 
-    # ---- time to reference SNR: plain FP32 form (zero start) and warm start --
+    # ---- time to reference SNR: plain (drop-in), batched engine, warm start --
     solve = None
     cpu = None
     if args.config == 'sara256' and not args.no_solve:
@@ -289,12 +310,14 @@ def main():
     elif not args.no_solve:
         from fhs import warmstart
         prm = dict(param, track_snr=True)
-        # the full plain solve fixes the reference SNR (and is the untimed
-        # warm-up); the times come from --solve-repeats further solves of the
-        # same deterministic sequence, each stopped once it reaches ref - 0.1 dB
-        _, plain = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds)
-        ref = plain['snr'][-1]
-        log(f'reference solve: {plain["iterations"]} iterations, {plain["wall"]:.1f} s, SNR {ref:.3f} dB')
+        # a full batched solve (the untimed warm-up; without the oracle fixture
+        # its converged SNR is the reference); the times come from
+        # --solve-repeats further solves of each form, each stopped once it
+        # reaches ref - 0.1 dB
+        _, full = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds)
+        ref = full['snr'][-1] if ref_fx is None else float(ref_fx['snr'][-1])
+        log(f'batched solve: {full["iterations"]} iterations, {full["wall"]:.1f} s, SNR {full["snr"][-1]:.3f} dB '
+            f'(reference {ref:.3f} dB)')
         tols = (0.1, 0.5, 1.0, 2.0, 3.0)
 
         def timed(**kw):
@@ -313,15 +336,41 @@ def main():
                 every[f'ref-{d}dB'] = [round(t, 4) for t in ts]
             return out, every
 
-        p_reach, p_all = reach(timed())
-        log('timed plain solves done')
-        solve = {'ref_snr_db': round(ref, 3), 'ref_definition': 'SNR of the plain run at convergence '
-                 '(3 reweighting steps, reference stopping rules); SNR = 20 log10(||X0||/||X0 - X||)',
-                 'timing': f'median of {max(1, args.solve_repeats)} solves after the full reference solve',
-                 'plain': {'iterations': plain['iterations'], 'wall_s': round(plain['wall'], 3),
-                           'converged': plain['converged'], 'reweights': plain['reweights'],
-                           'reach': p_reach, 'reach_s_each': p_all}}
+        if ref_fx is not None:
+            ref_def = ('converged SNR of the FP64 oracle on the same data (tests/golden/hs512x16_reference.npz: '
+                       f'{int(ref_fx["iterations"])} iterations, reweights at {[int(t) for t in ref_fx["reweights"]]}); '
+                       'SNR = 20 log10(||X0||/||X0 - X||)')
+        else:
+            ref_def = ('SNR of the batched GPU run at convergence (no FP64 oracle solve of this size; 3 reweighting '
+                       'steps, reference stopping rules); SNR = 20 log10(||X0||/||X0 - X||)')
+        # plain: the drop-in form (the reference-named entry points called one
+        # after the other each iteration, eager, zero start); batched: the
+        # engine (one hipGraph replay per iteration on three streams, stopping
+        # rule in the library); warm_start: the engine from the learned start
+        _, dropin = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets, time_limit=args.solve_seconds,
+                                   form='dropin')
+        log(f'drop-in solve: {dropin["iterations"]} iterations, {dropin["wall"]:.2f} s')
+        d_reach, d_all = reach(timed(form='dropin'))
+        b_reach, b_all = reach(timed())
+        log('timed plain and batched solves done')
+        solve = {'ref_snr_db': round(ref, 4), 'ref_definition': ref_def,
+                 'gpu_final_snr_db': round(full['snr'][-1], 4),
+                 'gpu_minus_ref_db': round(full['snr'][-1] - ref, 4),
+                 'oracle_reweights': None if ref_fx is None else [int(t) for t in ref_fx['reweights']],
+                 'oracle_iterations': None if ref_fx is None else int(ref_fx['iterations']),
+                 'timing': f'median of {max(1, args.solve_repeats)} solves after a full solve of each form',
+                 'plain': {'form': 'drop-in entry points per iteration (update_primal, update_dual_lowrankness, '
+                                   'update_dual_sparsity, update_dual_data_fidelity), eager, zero start',
+                           'iterations': dropin['iterations'], 'wall_s': round(dropin['wall'], 3),
+                           'converged': dropin['converged'], 'reweights': dropin['reweights'],
+                           'final_snr_db': round(dropin['snr'][-1], 4), 'reach': d_reach, 'reach_s_each': d_all},
+                 'batched': {'form': 'PDFB engine: one hipGraph replay per iteration, priors on two side streams, '
+                                     'stopping rule evaluated in the library, zero start',
+                             'iterations': full['iterations'], 'wall_s': round(full['wall'], 3),
+                             'converged': full['converged'], 'reweights': full['reweights'],
+                             'final_snr_db': round(full['snr'][-1], 4), 'reach': b_reach, 'reach_s_each': b_all}}
         wpath = warmstart.default_weights_path()
+        forms = ['batched']
         if os.path.exists(wpath) and not dr:
             ws = warmstart.WarmStart.load(ctx, wpath)
             ws.predict(warmstart.dirty_cube(meas))      # one untimed call: workspaces and code objects
@@ -338,45 +387,55 @@ def main():
             log(f'warm-start solve: {warm["iterations"]} iterations')
             w_reach, w_all = reach(timed(x_init=x0, weights_from_init=False), t_pred)
             log('timed warm-start solves done')
-            solve['warm_start'] = {'network': os.path.relpath(wpath, ROOT), 'predict_s': round(t_pred, 4),
+            solve['warm_start'] = {'form': 'PDFB engine from the learned warm start (inference time included), '
+                                           'unit first weights',
+                                   'network': os.path.relpath(wpath, ROOT), 'predict_s': round(t_pred, 4),
                                    'x_init_snr_db': round(x0_snr, 3),
-                                   'iterations': warm['iterations'], 'final_snr_db': round(warm['snr'][-1], 3),
+                                   'iterations': warm['iterations'], 'final_snr_db': round(warm['snr'][-1], 4),
                                    'reach': w_reach, 'reach_s_each': w_all}
-            # the chosen form per tolerance: whichever start reaches it first
-            solve['chosen'] = {}
+            forms.append('warm_start')
+        # speed-up of each GPU form over the plain form, per tolerance
+        solve['speedup_vs_plain'] = {}
+        for f in forms:
+            sp = {}
             for d in tols:
                 key = f'ref-{d}dB'
-                p, w = solve['plain']['reach'][key], solve['warm_start']['reach'][key]
-                if p is None and w is None:
-                    solve['chosen'][key] = None
-                elif w is not None and (p is None or w['s'] < p['s']):
-                    solve['chosen'][key] = {'form': 'warm_start', **w}
-                else:
-                    solve['chosen'][key] = {'form': 'plain', **p}
+                p_, q_ = solve['plain']['reach'][key], solve[f]['reach'][key]
+                sp[key] = None if (p_ is None or q_ is None) else round(p_['s'] / q_['s'], 3)
+            solve['speedup_vs_plain'][f] = sp
 
-    # ---- CPU baseline: FP64 oracle, bounded sample -------------------------
+    # ---- CPU baseline: FP64 oracle on the host cores -----------------------
     if not args.no_cpu and args.config != 'sara256':
-        from threadpoolctl import threadpool_limits
-        from oracle.solvers import OracleProblem, hypersara as hs_ref
+        from oracle.nufft import uniform_precond_weights
+        from oracle.solvers import OracleProblem, pdfb
+        nproc = os.cpu_count() or 1
+        threads = min(16, nproc)     # the box's CPU share for one GPU
         blocks = []
         for l in range(prob.L):
             b = np.concatenate([[0], np.cumsum(prob.blocks[l])])
             blocks.append([(int(b[i]), int(b[i + 1])) for i in range(len(prob.blocks[l]))])
-        pU = [np.asarray(p, dtype=np.float64) for p in meas.pU_host]
-        with threadpool_limits(limits=1):
-            op = OracleProblem(prob.Ny, prob.Nx, meas.u, meas.v, meas.y_host, pU, blocks, prob.epsilon,
-                               g_scale=[1.0 / s for s in prob.sigma_noise])
-            sig, sig_bar = solvers.noise_levels_hs(prob.Ny, prob.Nx, prob.L, np.ones(prob.L), son)
-            pr = dict(solvers.DEFAULT_PARAM)
-            pr.update({'nu2': nu2, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar,
-                       'alpha': 1.0})
-            n_cpu = 1
-            tc = time.perf_counter()
-            hs_ref(op, pr, max_iter_total=n_cpu)
-            tcpu = time.perf_counter() - tc
-        cpu = {'value': round(n_cpu / tcpu, 5), 'unit': 'iters/s', 'cores': 1, 'kind': 'port',
-               'sample': f'{n_cpu} HyperSARA iteration(s) of the same 512x512x16 problem by the FP64 '
-                         f'numpy oracle, single thread ({tcpu:.1f} s)'}
+        pU = [uniform_precond_weights(prob.u[l], prob.v[l], 2 * prob.Ny, 2 * prob.Nx) for l in range(prob.L)]
+        op = OracleProblem(prob.Ny, prob.Nx, prob.u, prob.v, prob.y, pU, blocks, prob.epsilon,
+                           g_scale=[1.0 / s for s in prob.sigma_noise], threads=threads)
+        sig, sig_bar = solvers.noise_levels_hs(prob.Ny, prob.Nx, prob.L, np.ones(prob.L), son)
+        pr = dict(solvers.DEFAULT_PARAM)
+        pr.update({'nu2': nu2, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar, 'alpha': 1.0})
+        n_cpu = 10
+        tc = time.perf_counter()
+        pdfb('hs', op, pr, max_iter_total=n_cpu, threads=threads)
+        tcpu = time.perf_counter() - tc
+        log(f'CPU baseline: {n_cpu} oracle iterations on {threads} threads in {tcpu:.1f} s')
+        cpu = {'value': round(n_cpu / tcpu, 5), 'unit': 'iters/s', 'cores': threads, 'nproc': nproc,
+               'kind': 'port',
+               'sample': f'{n_cpu} HyperSARA iterations of the same 512x512x16 problem by the FP64 numpy oracle '
+                         f'(channels on a pool of {threads} threads, BLAS threaded), {tcpu:.1f} s'}
+        if ref_fx is not None:
+            from fhs import warmstart
+            k = warmstart.iterations_to(list(ref_fx['snr']), float(ref_fx['snr'][-1]) - 0.1)
+            if k is not None:
+                cpu['time_to_ref-0.1dB_s'] = round(k * tcpu / n_cpu, 1)
+                cpu['time_to_ref-0.1dB_note'] = (f'{k} oracle iterations to the reference - 0.1 dB (from the '
+                                                 f'reference solve) times the measured seconds per iteration')
 
     out = {
         'metric': METRIC,
@@ -410,7 +469,8 @@ def main():
                    'nnz_G_total': int(sum(g.nnz for g in (meas.meas.G if dr else meas.G))),
                    **({'active_modes': int(sum(f.n_modes for f in meas.forms)),
                        'nnz_H_lower_total': int(sum(f.H.nnz for f in meas.forms))} if dr else {}),
-                   'setup_s': round(setup_s, 2)},
+                   'setup_s': round(setup_s, 2),
+                   'opnorm_rel_gap_gpu_vs_fp64': opnorm_gap},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1),
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
                      'traffic': traffic, 'avg_launch_ms': round(avg_ms, 5),

@@ -225,6 +225,9 @@ class PdfbEngine:
         s22 = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma22, dtype=np.float32), (L,)))
         eps = np.ascontiguousarray(np.asarray(epsilon, dtype=np.float64))
         dr = self.dr if dr is None else dr
+        # kept for the drop-in form (dropin_iterate), which passes them per call
+        self._prm = dict(beta1=float(beta1), sigma00=float(sigma00), sigma11=float(sigma11), sigma22=s22,
+                         epsilon=eps, fb=(int(fb_max), int(fb_min), float(fb_eps)), dr=bool(dr))
         self.ctx.check(lib().fhs_pdfb_set_params(
             self.h, th, float(beta1), float(sigma00), float(sigma11), s22.ctypes.data_as(C.POINTER(C.c_float)),
             eps.ctypes.data_as(C.POINTER(C.c_double)), int(fb_max), int(fb_min), float(fb_eps), int(dr)),
@@ -254,6 +257,38 @@ class PdfbEngine:
             int(n_max), float(err2_stop), float(seconds_max), out.ctypes.data_as(C.POINTER(C.c_double)),
             C.byref(n), C.byref(conv)), 'pdfb_run')
         return out[:6 * n.value].reshape(n.value, 6), bool(conv.value)
+
+    def dropin_iterate(self):
+        """One iteration in the drop-in form: the reference-named entry points
+        called one after the other on the engine's buffers, eagerly on the
+        context stream, each returning its host scalars -- the sequence a
+        MATLAB solver makes through the MEX drop-ins
+        (``src/hs/hyperSARA.m:574-613``: primal update, low-rank and sparsity
+        dual steps, ``update_dual_data_fidelity``).  Same kernels and the
+        same arithmetic as ``iterate``; no graph, no concurrent streams, no
+        fused reductions.  Returns the four sums of ``iterate``."""
+        ctx, P, m = self.ctx, self._prm, self.meas
+        L, n = m.L, m.L * m.Ny * m.Nx
+        rel, nrm = C.c_double(), C.c_double()
+        ctx.check(lib().fhs_update_primal(ctx.h, _ptr(self.x), _ptr(self.g1), _ptr(self.g2), _ptr(self.g0),
+                                          _ptr(self.xhat), n, C.byref(rel), C.byref(nrm)), 'update_primal')
+        if self.lowrank is not None:
+            th = self._th
+            ctx.check(lib().fhs_update_dual_lowrankness(ctx.h, self.lowrank.h, _ptr(self.v0), _ptr(self.xhat),
+                                                        th.ctypes.data_as(C.POINTER(C.c_double)), 1.0,
+                                                        P['sigma00'], _ptr(self.g0)), 'update_dual_lowrankness')
+        ctx.check(lib().fhs_update_dual_sparsity(ctx.h, self.sara.h, _ptr(self.v1), _ptr(self.xhat), L,
+                                                 _ptr(self.weights1), P['beta1'], P['sigma11'], _ptr(self.g1)),
+                  'update_dual_sparsity')
+        nr = np.zeros(m.D.nblocks)
+        gnr, neps = C.c_double(), C.c_double()
+        fb_max, fb_min, fb_eps = P['fb']
+        ctx.check(lib().fhs_update_dual_data_fidelity(
+            ctx.h, m.D.h, _ptr(self.v2), _ptr(m.y), _ptr(self.x), _ptr(self.gx_old), _ptr(self.proj),
+            _ptr(m.pU), P['epsilon'].ctypes.data_as(C.POINTER(C.c_double)), fb_max, fb_min, fb_eps,
+            P['sigma22'].ctypes.data_as(C.POINTER(C.c_float)), int(P['dr']), _ptr(self.Lambda), _ptr(self.g2),
+            nr.ctypes.data_as(C.POINTER(C.c_double)), C.byref(gnr), C.byref(neps)), 'update_dual_data_fidelity')
+        return np.array([rel.value, nrm.value, gnr.value, neps.value])
 
     def set_truth(self, X0):
         """Reference cube (L, Nx, Ny) float32 on the GPU, or None: the primal
@@ -302,7 +337,8 @@ def snr_db(X0, x):
 
 
 def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, graph=True,
-           max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True, stop_snr=None):
+           max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True, stop_snr=None,
+           form='engine'):
     """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'.
 
     With a nonzero ``x_init`` the reference derives the first weights from it
@@ -313,7 +349,9 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     Without a ``callback`` the iterations of a reweighting cycle run inside
     the library (``fhs_pdfb_run``: the same stopping rule, no host round trip
     per iteration); ``stop_snr`` (with X0) ends the solve at the first
-    iteration whose SNR reaches it."""
+    iteration whose SNR reaches it.  ``form='dropin'`` runs every iteration
+    as the sequence of reference-named entry points instead
+    (``PdfbEngine.dropin_iterate``; SNR by a separate reduction)."""
     prm = dict(DEFAULT_PARAM)
     prm.update(param or {})
     ctx = meas.ctx
@@ -414,6 +452,34 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     # variation on the data fidelity (src/fhs/facetHyperSARA.m:877-881);
     # SARA (src/sara/sara.m:666-670) and HyperSARA (src/hs/hyperSARA.m:682-686) do not
     rule = 1 if kind == 'fhs' else 0
+    if callback is None and form == 'dropin':
+        t = 0
+        while t < max_iter:
+            sc = eng.dropin_iterate()
+            t += 1
+            tc = time.perf_counter() - t0
+            rel_val = sqrt(sc[0]) / sqrt(sc[1]) if sc[1] > 0 else float('nan')
+            nres, neps = sqrt(sc[2]), sqrt(sc[3])
+            info['rel_val'].append(rel_val)
+            info['norm_res'].append(nres)
+            info['norm_eps'].append(neps)
+            if track:
+                e2 = torch.linalg.vector_norm((eng._truth - eng.x).double()).item() ** 2
+                info['snr'].append(20 * np.log10(nX0 / sqrt(e2)) if e2 > 0 else float('inf'))
+            info['time'].append(time.perf_counter() - t0)
+            if not np.isfinite(nres):
+                raise RuntimeError('dropin: non-finite data residual')
+            if pdfb_converged(rule, t - last_step, rel_val, nres, neps, prm):
+                if end_of_cycle(t):
+                    break
+                continue
+            if track and stop_snr is not None and info['snr'][-1] >= stop_snr:
+                break
+            if time_limit is not None and tc > time_limit:
+                break
+        info['iterations'] = t
+        info['wall'] = time.perf_counter() - t0
+        return eng, info
     if callback is None:
         # whole cycles inside the library
         err2_stop = -1.0

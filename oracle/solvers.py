@@ -36,11 +36,19 @@ class OracleProblem:
     ranges; epsilon: per-channel list of block bounds.
     """
 
-    def __init__(self, Ny, Nx, u, v, y, pU, blocks, epsilon, g_scale=None, nufft_kwargs=None):
+    def __init__(self, Ny, Nx, u, v, y, pU, blocks, epsilon, g_scale=None, nufft_kwargs=None, threads=1):
         self.Ny, self.Nx = Ny, Nx
         self.L = len(u)
         kw = nufft_kwargs or {}
-        self.ops = [NufftOracle(u[l], v[l], Ny, Nx, **kw) for l in range(self.L)]
+
+        def build(l):
+            return NufftOracle(u[l], v[l], Ny, Nx, **kw)
+        if threads > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(threads) as ex:
+                self.ops = list(ex.map(build, range(self.L)))
+        else:
+            self.ops = [build(l) for l in range(self.L)]
         if g_scale is not None:     # whitening G <- G / sigma (util_gen_measurements_snr_new.m:78)
             for l, op in enumerate(self.ops):
                 op.G = op.G * g_scale[l]

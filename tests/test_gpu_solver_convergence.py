@@ -106,3 +106,46 @@ def test_fb_exchange_timeout_is_reported(ctx):
         eng.run(0, 0, solvers.DEFAULT_PARAM, 5)
     out, _ = eng.run(0, 0, solvers.DEFAULT_PARAM, 3)
     assert np.all(np.isfinite(out))
+
+
+@pytest.mark.parametrize('kind', ['hs', 'fhs'])
+def test_dropin_form_matches_engine(ctx, kind):
+    """The drop-in form (the reference-named entry points called one after
+    the other every iteration) takes the engine's decisions and iterates."""
+    from fhs import solvers, synth
+    from fhs.lowrank import facet_geometry
+    prob = synth.make_problem(64, 64, 4, 4000, 2, seed=37, n_sources=30)
+    synth.add_host_data(prob)
+    meas = solvers.Measurement(ctx, prob, precond=True)
+    son, nu2 = meas.op_norm(False), meas.op_norm(True)
+    p = {'squared_operator_norm': son, 'nu2': nu2, 'track_snr': True, 'reweighting_max_iter': 2}
+    facets = facet_geometry(64, 64, 2, 2, (0.25, 0.25)) if kind == 'fhs' else None
+    _, a = solvers._solve(kind, meas, p, X0=prob.X0, facets=facets)
+    eb, b = solvers._solve(kind, meas, p, X0=prob.X0, facets=facets, form='dropin')
+    assert a['reweights'] and a['reweights'] == b['reweights'] and a['iterations'] == b['iterations']
+    assert np.allclose(a['snr'], b['snr'], rtol=0, atol=1e-5)
+    assert np.allclose(np.array(a['rel_val']), np.array(b['rel_val']), rtol=1e-4, equal_nan=True)
+
+
+def test_hs512x16_solve_matches_fp64_reference(ctx):
+    """The benchmark workload (HyperSARA 512x512x16, 3 reweighting steps)
+    solved on the GPU ends within 0.1 dB of the FP64 oracle's converged SNR
+    on the same data, with the oracle's reweighting iterations (fixture of
+    tests/golden/make_hs512_reference.py)."""
+    path = os.path.join(GOLDEN, 'hs512x16_reference.npz')
+    if not os.path.exists(path):
+        pytest.skip('no FP64 reference fixture for the 512x512x16 workload')
+    from fhs import solvers, synth
+    fx = np.load(path)
+    prob = synth.make_problem(512, 512, 16, 200000, 4, seed=0)
+    synth.add_host_data(prob)
+    meas = solvers.Measurement(ctx, prob, precond=True)
+    p = {'squared_operator_norm': fx['son'], 'nu2': fx['nu2'], 'track_snr': True, 'reweighting_max_iter': 3}
+    _, info = solvers.hyper_sara(meas, p, X0=prob.X0)
+    ref_rw = [int(t) for t in fx['reweights']]
+    print(f'GPU {info["iterations"]} iterations, reweights {info["reweights"]}, SNR {info["snr"][-1]:.4f}; '
+          f'oracle {int(fx["iterations"])}, {ref_rw}, {fx["snr"][-1]:.4f}', flush=True)
+    assert info['converged'] and bool(fx['converged'])
+    assert abs(info['snr'][-1] - float(fx['snr'][-1])) <= 0.1
+    assert len(info['reweights']) == len(ref_rw)
+    assert all(abs(a - b) <= 2 for a, b in zip(info['reweights'], ref_rw))
