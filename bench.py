@@ -63,23 +63,35 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--config', choices=('hs512', 'hs512dr', 'fhs1024', 'fhs2048', 'sara256'), default='hs512',
-                    help='hs512: HyperSARA 512x512x16 (the headline workload); fhs1024: one 16-channel spectral '
+    ap.add_argument('--w-max', type=float, default=None,
+                    help='w-term: base w uniform in [0, W] wavelengths (fhs2048: 200), field of view --fov')
+    ap.add_argument('--fov', type=float, default=0.08, help='field of view of the w-term (direction cosines)')
+    ap.add_argument('--config', choices=('fhs2048x64', 'hs512', 'hs512dr', 'fhs1024', 'fhs2048', 'sara256'),
+                    default='fhs2048x64',
+                    help='fhs2048x64: Faceted HyperSARA 2048x2048x64 (the headline workload) as C = 4 spectral '
+                         'sub-cubes of 16 channels solved on one GPU, 6x6 facets with 50%% overlap, 1M visibilities '
+                         'per channel, w-term (w in [0, 200] wavelengths, 0.08 rad field); hs512: HyperSARA '
+                         '512x512x16 against the FP64 oracle\'s converged SNR; fhs1024: one 16-channel spectral '
                          'sub-cube of Faceted HyperSARA 1024x1024x32 (C = 2), 4x4 nuclear-norm facets with 25%% '
                          'overlap, 300k-500k visibilities per channel; fhs2048: one 16-channel spectral sub-cube '
                          'of Faceted HyperSARA 2048x2048x64 (C = 4), 6x6 facets with 50%% overlap, 1M visibilities '
-                         'per channel, no w-term; sara256: SARA 256x256, 65,536 '
+                         'per channel, w-term with w in [0, 200] wavelengths over a 0.08 rad field (--w-max 0 '
+                         'turns it off); sara256: SARA 256x256, 65,536 '
                          'visibilities, 2 blocks, 300 iterations, against the FP64 oracle run to the same end '
                          '(the CPU-runnable case: its SNR is the reference); hs512dr: the hs512 problem in its '
                          'dimensionality-reduced form (holographic H = tril(G\'G) per channel, no preconditioning, '
                          'imaging/imaging.m:619-631), its own converged SNR the reference')
     a = ap.parse_args()
+    if a.config == 'fhs2048x64':
+        return a
     if a.config == 'fhs1024':
         a.size, a.channels, a.vis = 1024, 16, 500000
         a.no_cpu = True     # one FP64 oracle iteration of this size takes minutes
     if a.config == 'fhs2048':
         a.size, a.channels, a.vis = 2048, 16, 1000000
         a.no_cpu = True
+        if a.w_max is None:
+            a.w_max = 200.0
     if a.config == 'sara256':
         a.size, a.channels, a.vis = 256, 1, 65536
     if a.config == 'hs512dr':
@@ -98,14 +110,15 @@ def algorithmic_bytes(meas, sara_s):
     nf = meas.nufft
     M = [g.M for g in meas.G]
     N = meas.Ny * meas.Nx
-    row = 8 * (nf.Ky + nf.Kx) + 4
+    Ky, Kx = getattr(meas.G[0], 'Ky', nf.Ky), getattr(meas.G[0], 'Kx', nf.Kx)
+    row = 8 * (Ky + Kx) + 4
     fwd = sum(row * m + 8 * No + 24 * m for m in M)                      # coefs, Fx, gx_old r/w, r2
     adj = 0
     for a in meas.anchor:
         a = a.astype(np.int64)
         ay, ax = a % nf.Kty, a // nf.Kty
-        ny = 1 + (ay // 32 != ((ay + nf.Ky - 1) % nf.Kty) // 32)
-        nx = 1 + (ax // 32 != ((ax + nf.Kx - 1) % nf.Ktx) // 32)
+        ny = 1 + (ay // 32 != ((ay + Ky - 1) % nf.Kty) // 32)
+        nx = 1 + (ax // 32 != ((ax + Kx - 1) % nf.Ktx) // 32)
         entries = int(np.sum(ny * nx))
         adj += (row - 4 + 8 + 4) * len(a) + 8 * entries + 4 * (No // 1024 + 1) + 8 * No
     return {
@@ -188,6 +201,210 @@ def sara_against_oracle(args, solvers, meas, prob, son, nu2, n_iter=300):
     return solve, cpu
 
 
+def cpu_sample_cube(cfg, prob, meas, facets, threads):
+    """The FP64 oracle's cost of one PDFB iteration of the whole cube,
+    from a bounded sample: one channel's data step at full size (A, At and
+    the ellipsoid projection of its 4 blocks) with the first 1/16 of its
+    w-projected rows, the channel's wavelet analysis and synthesis, and one
+    facet's low-rank prox (SVD of the facet's 16 channels); scaled as
+    channels x (transforms + 16 x row-proportional part) + facets x prox."""
+    import torch
+    import scipy.sparse as sp
+    from oracle.nufft import NufftOracle
+    from oracle.prox import nuclear_prox_dual, solver_proj_elipse_fb
+    from oracle.wavelets import SaraDictionary
+    N = cfg['N']
+    C, L = cfg['subcubes'], cfg['channels']
+    frac = 16
+    m = len(prob.u[0]) // frac
+    nf = meas.nufft
+    a, cy, cx = nf.separable_wproj_gpu(meas.ctx, prob.u[0][:m], prob.v[0][:m], prob.w[0][:m], cfg['fov'])
+    Ky, Kx = cy.shape[1], cx.shape[1]
+    ky = (a % nf.Kty)[:, None] + np.arange(Ky)[None, :]
+    kx = (a // nf.Kty)[:, None] + np.arange(Kx)[None, :]
+    cols = ((ky % nf.Kty)[:, :, None] + nf.Kty * (kx % nf.Ktx)[:, None, :]).reshape(m, -1)
+    vals = (cy.astype(np.complex128)[:, :, None] * cx.astype(np.complex128)[:, None, :]).reshape(m, -1)
+    G = sp.csr_matrix((vals.ravel(), (np.repeat(np.arange(m), Ky * Kx), cols.ravel())), shape=(m, nf.No))
+    GH = G.conj().T.tocsr()
+    op = NufftOracle(np.zeros(1), np.zeros(1), N, N)
+    rng = np.random.default_rng(0)
+    x = rng.random((N, N))
+    y = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    pU = np.ones(m)
+    from threadpoolctl import threadpool_limits
+    lim = threadpool_limits(threads)
+    t0 = time.perf_counter()
+    Fx = op.A(x)                                   # transforms: A and At
+    _ = np.real(op.At(Fx))
+    t_tr = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = G @ Fx                                     # rows: G, the ellipsoid projection, G'
+    z, _ = solver_proj_elipse_fb(np.zeros(m), r, y, pU, 0.5 * np.linalg.norm(r - y), np.zeros(m, complex), 20, 1, 1e-8)
+    _ = GH @ (r - z)
+    t_rows = time.perf_counter() - t0
+    sara = SaraDictionary(N, N)
+    t0 = time.perf_counter()
+    c = sara.psit(x)
+    _ = sara.psi(c)
+    t_wl = time.perf_counter() - t0
+    P = facets['py'][0] * facets['px'][0]
+    Z = rng.standard_normal((max(facets['py']) * max(facets['px']), L // C))
+    t0 = time.perf_counter()
+    nuclear_prox_dual(Z, np.ones(L // C))
+    t_fac = time.perf_counter() - t0
+    lim.restore_original_limits()
+    nq = len(facets['oy'])
+    t_iter = L * (t_tr + frac * t_rows + t_wl) + C * nq * t_fac
+    return {'value': round(1.0 / t_iter, 6), 'unit': 'iters/s', 'cores': threads, 'nproc': os.cpu_count(),
+            'kind': 'port',
+            'sample': (f'FP64 numpy oracle on a bounded sample of the workload: one channel at 2048x2048 -- A and '
+                       f'At ({t_tr:.2f} s), G, the 20-step ellipsoid projection and G\' on 1/{frac} of its '
+                       f'{len(prob.u[0])} w-projected rows ({Ky}x{Kx} taps: {t_rows:.2f} s), the SARA analysis '
+                       f'and synthesis ({t_wl:.2f} s) -- and one facet\'s low-rank prox ({t_fac:.2f} s); one cube '
+                       f'iteration = {L} x (transforms + {frac} x rows + wavelets) + {C} x {nq} x prox '
+                       f'= {t_iter:.0f} s, one core')}
+
+
+def cube_main(args):
+    """The 2048x2048x64 cube as C independent sub-cube solves on one GPU."""
+    import torch
+    import fhs
+    from fhs import solvers, warmstart
+    from fhs.cube import CUBE_2048x64, build_subcube
+    cfg = dict(CUBE_2048x64)
+    if args.w_max is not None:
+        cfg['w_max'] = args.w_max
+    cfg['fov'] = args.fov
+    ctx = fhs.default_context(0)
+    C = cfg['subcubes']
+    t_setup = time.perf_counter()
+    subs = []
+    for c in range(C):
+        subs.append(build_subcube(ctx, cfg, c))
+        log(f'sub-cube {c} set up ({time.perf_counter() - t_setup:.0f} s)')
+    setup_s = time.perf_counter() - t_setup
+    # ---- throughput: one PDFB iteration of every sub-cube per step ----------
+    engines = []
+    for prob, meas, facets, param in subs:
+        holder = {}
+
+        def grab(t, eng, info, holder=holder):
+            holder['eng'] = eng
+            return t < args.warmup
+        solvers._solve('fhs', meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1)
+        engines.append(holder['eng'])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for eng in engines:
+            eng.iterate(1)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cube_per_s = args.steps / elapsed
+    log(f'{args.steps} timed cube iterations ({C} sub-cubes each): {cube_per_s:.3f} iterations/s')
+    # ---- per-kernel timing on sub-cube 0 ----------------------------------
+    eng = engines[0]
+    eng.set_concurrent(False)
+    ctx.timing_reset()
+    ctx.set_timing(True)
+    n_timed = 5
+    eng.iterate(n_timed)
+    ctx.set_timing(False)
+    rep = ctx.timing_report()
+    ctx.timing_reset()
+    meas0 = subs[0][1]
+    alg = algorithmic_bytes(meas0, eng.sara.s)
+    dominant = max((k for k in rep if k in alg), key=lambda k: rep[k][0])
+    avg_ms = rep[dominant][0] / rep[dominant][1]
+    achieved = alg[dominant] / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {'ms_per_launch': rep[k][0] / rep[k][1], 'launches_per_iter': rep[k][1] / n_timed,
+                   'GBps': (alg[k] / (rep[k][0] / rep[k][1] * 1e-3) / 1e9) if k in alg else None}
+               for k in rep}
+    del engines, eng
+    torch.cuda.synchronize()
+    # ---- time to reference SNR: per sub-cube, plain (drop-in) and batched --
+    solve = None
+    fx_path = os.path.join(ROOT, 'tests', 'golden', 'fhs2048x64_reference.npz')
+    default_cfg = dict(CUBE_2048x64, fov=args.fov) == cfg and args.seed == 0
+    if not args.no_solve and os.path.exists(fx_path) and default_cfg:
+        fx = np.load(fx_path)
+        tols = (0.1, 0.5, 1.0, 2.0, 3.0)
+        per = []
+        for c, (prob, meas, facets, param) in enumerate(subs):
+            ref = float(fx[f'snr_{c}'][-1])
+            prm = dict(param, track_snr=True)
+            row = {'ref_snr_db': round(ref, 4), 'reference_iterations': int(fx[f'iterations_{c}']),
+                   'reference_reweights': [int(t) for t in fx[f'reweights_{c}']]}
+            for form in ('dropin', 'engine'):
+                _, info = solvers._solve('fhs', meas, prm, X0=prob.X0, facets=facets, form=form,
+                                         stop_snr=ref - min(tols), time_limit=args.solve_seconds)
+                reach = {}
+                for d in tols:
+                    k = warmstart.iterations_to(info['snr'], ref - d)
+                    reach[f'ref-{d}dB'] = None if k is None else {'iterations': k, 's': round(info['time'][k - 1], 4)}
+                row['plain' if form == 'dropin' else 'batched'] = reach
+            per.append(row)
+            log(f'sub-cube {c}: timed solves done')
+        total = {}
+        for f in ('plain', 'batched'):
+            total[f] = {}
+            for d in tols:
+                key = f'ref-{d}dB'
+                vals = [r[f][key] for r in per]
+                total[f][key] = None if any(v is None for v in vals) else round(sum(v['s'] for v in vals), 4)
+        speed = {key: (None if total['plain'][key] is None or total['batched'][key] is None else
+                       round(total['plain'][key] / total['batched'][key], 3)) for key in total['plain']}
+        solve = {'ref_definition': ('per sub-cube, the converged SNR of the plain GPU solve with the reference '
+                                    'stopping rules and 3 reweighting steps (tests/golden/fhs2048x64_reference.npz; '
+                                    'no FP64 solve of this size is feasible -- the solver loop is checked against '
+                                    'the FP64 oracle at small sizes); SNR = 20 log10(||X0||/||X0 - X||)'),
+                 'timing': 'the sub-cubes are independent solves run one after the other: the cube reaches a '
+                           'tolerance when every sub-cube has, at the sum of their times',
+                 'plain_form': 'drop-in entry points per iteration, eager, zero start',
+                 'batched_form': 'PDFB engine (one hipGraph replay per iteration, priors on side streams, stopping '
+                                 'rule in the library), zero start',
+                 'warm_start': 'not run at this size (the shipped network is trained on 512x512x16 cubes)',
+                 'subcubes': per, 'total_s': total, 'speedup_batched_vs_plain': speed}
+    # ---- CPU baseline: bounded oracle sample --------------------------------
+    cpu = None
+    if not args.no_cpu:
+        threads = 1         # numpy's FFT and the wavelet filters run on one core; BLAS limited to match
+        prob, meas, facets, _ = subs[0]
+        cpu = cpu_sample_cube(cfg, prob, meas, facets, threads)
+        log('CPU baseline sample done')
+    nnz = int(sum(g.nnz for (_, m, _, _) in subs for g in m.G))
+    Ky = subs[0][1].G[0].Ky
+    out = {
+        'metric': METRIC,
+        'value': round(cube_per_s, 4),
+        'unit': 'iters/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(1e3 * elapsed / args.steps, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 (f64 norms, f64 Gram/eig)',
+        'data': 'synthetic',
+        'config': {'workload': (f'Faceted HyperSARA {cfg["N"]}x{cfg["N"]}x{cfg["channels"]} as C = {C} spectral '
+                                f'sub-cubes of {cfg["channels"] // C} channels solved on one GPU (one step = one PDFB '
+                                f'iteration of every sub-cube), 6x6 nuclear-norm facets, 50% overlap, '
+                                f'{cfg["vis"]} vis/channel, 4 blocks/channel, w-term (w in [0, {cfg["w_max"]:g}] '
+                                f'wavelengths, {cfg["fov"]} rad field, w-projected separable rows of {Ky}x{Ky} taps)'),
+                   'Mtot': int(sum(m.D.Mtot for (_, m, _, _) in subs)), 'nnz_G_total': nnz,
+                   'setup_s': round(setup_s, 2)},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
+                     'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
+                     'avg_launch_ms': round(avg_ms, 5), 'algorithmic_bytes_per_launch': int(alg[dominant]),
+                     'measured_on': 'sub-cube 0 (16 channels)'},
+        'kernels': kernels,
+        'solve': solve,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -208,6 +425,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes_read': n, 'copy_bytes_written': n, 'repeats': 3}))
         return
+    if args.config == 'fhs2048x64':
+        return cube_main(args)
     ctx = fhs.default_context(0)
     t_setup = time.perf_counter()
     facets = None
@@ -219,7 +438,7 @@ def main():
     elif args.config == 'fhs2048':
         from fhs.lowrank import facet_geometry
         prob = synth.make_problem(args.size, args.size, args.channels, args.vis, 4, seed=args.seed,
-                                  flag_range=None)
+                                  flag_range=None, w_max=args.w_max or None, fov=args.fov)
         facets = facet_geometry(args.size, args.size, 6, 6, (0.5, 0.5))
     elif args.config == 'sara256':
         prob = synth.make_problem(args.size, args.size, 1, args.vis, 2, seed=args.seed)
@@ -460,7 +679,9 @@ def main():
                    if facets is None else
                    (f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 4 spectral sub-cubes '
                     f'of {args.size}x{args.size}x{4 * args.channels}), 6x6 nuclear-norm facets, 50% overlap, '
-                    f'{args.vis} vis/channel, no w-term, 4 blocks/channel'
+                    f'{args.vis} vis/channel, '
+                    + (f'w-term (w in [0, {args.w_max:g}] wavelengths, field of view {args.fov} rad, '
+                       f'w-projected separable rows of {meas.G[0].Ky} taps per axis)' if args.w_max else 'no w-term') + ', 4 blocks/channel'
                     if args.config == 'fhs2048' else
                     f'Faceted HyperSARA {args.size}x{args.size}x{args.channels} (one of C = 2 spectral sub-cubes '
                     f'of {args.size}x{args.size}x{2 * args.channels}), 4x4 nuclear-norm facets, 25% overlap, '

@@ -110,6 +110,13 @@ int guard(fhs_ctx* ctx, F&& body) {
 
 inline void check_launch() { FHS_HIP(hipGetLastError()); }
 
+// Lets `kernel` take `bytes` of dynamic LDS when that is above the 64 KiB
+// default (a CU has 160 KiB).  Cheap to repeat; done before each launch
+// that may need it.
+inline void allow_dynamic_lds(const void* kernel, size_t bytes) {
+  if (bytes > 65536) FHS_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 // ---------------------------------------------------------------- device --

@@ -28,6 +28,7 @@ struct fhs_data {
   int tile_grid = 0;                   // max tile work items over tile-gridded channels
   int red_grid = 0;                    // max split tiles over tile-gridded channels
   size_t tile_lds = 0;                 // LDS bytes of a tile workgroup (max over channels)
+  int kmax = 0;                        // slots per axis of the tiled channels' separable rows
   int fwd_grid = 0;                    // max forward work items over tile-gridded channels
   bool any_rows = false;               // some channel without tile gridding (row kernels)
   int64_t No = 0;
@@ -583,7 +584,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint(const GDev* __rest
 
 // g2_l = G_l' (lam .* v2_l) for the channels with a tile-gridded separable G
 // (one workgroup per grid tile; never the DR form, whose H is square).
-template <bool LAM>
+template <bool LAM, int KM>
 __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_tiled(const GDev* __restrict__ gd,
                                                                     const int64_t* __restrict__ row_off,
                                                                     const float2* __restrict__ v2,
@@ -597,9 +598,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_adjoint_tiled(const GDev* 
   const float2* __restrict__ v = v2 + row_off[l];
   const float* __restrict__ lm = LAM ? lam + row_off[l] : nullptr;
   if (LAM)
-    g_adjoint_tile(g, u, [=](int r) { return cscale(v[r], lm[r]); }, g2 + (size_t)l * No, t);
+    g_adjoint_tile<KM>(g, u, [=](int r) { return cscale(v[r], lm[r]); }, g2 + (size_t)l * No, t);
   else
-    g_adjoint_tile(g, u, [=](int r) { return v[r]; }, g2 + (size_t)l * No, t);
+    g_adjoint_tile<KM>(g, u, [=](int r) { return v[r]; }, g2 + (size_t)l * No, t);
 }
 
 // sums of the partial tiles of the tiles split over several work items
@@ -622,12 +623,15 @@ void launch_data_adjoint(fhs_ctx* ctx, const fhs_data* D, const float2* v2, cons
     return;
   }
   if (D->tile_grid > 0) {
-    if (lam)
-      k_data_adjoint_tiled<true><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, D->tile_lds, ctx->stream>>>(
-          D->gdev, D->d_row_off, v2, lam, D->No, D->g2, L, D->tile_grid);
+    auto go = [&](auto kern) {
+      allow_dynamic_lds((const void*)kern, D->tile_lds);
+      kern<<<xcd_grid(L, D->tile_grid), ADJ_THREADS, D->tile_lds, ctx->stream>>>(D->gdev, D->d_row_off, v2, lam,
+                                                                                 D->No, D->g2, L, D->tile_grid);
+    };
+    if (D->kmax == 8)
+      lam ? go(k_data_adjoint_tiled<true, 8>) : go(k_data_adjoint_tiled<false, 8>);
     else
-      k_data_adjoint_tiled<false><<<xcd_grid(L, D->tile_grid), ADJ_THREADS, D->tile_lds, ctx->stream>>>(
-          D->gdev, D->d_row_off, v2, nullptr, D->No, D->g2, L, D->tile_grid);
+      lam ? go(k_data_adjoint_tiled<true, 16>) : go(k_data_adjoint_tiled<false, 16>);
     check_launch();
     if (D->red_grid > 0) {
       k_data_adjoint_reduce<<<xcd_grid(L, D->red_grid), ADJ_THREADS, 0, ctx->stream>>>(D->gdev, D->No, D->g2, L,
@@ -662,19 +666,20 @@ __global__ __launch_bounds__(256) void k_data_apply(const GDev* __restrict__ gd,
 
 // the same two operations for the channels with a tile-gridded separable G:
 // one workgroup per forward work item, grid window staged in LDS
+template <int KM>
 __global__ __launch_bounds__(ADJ_THREADS) void k_data_forward_tiled(const GDev* __restrict__ gd,
                                                                     const int64_t* __restrict__ row_off,
                                                                     const float2* __restrict__ Fx, int64_t No,
                                                                     float2* __restrict__ gx_old,
                                                                     float2* __restrict__ r2, int L, int units) {
-  __shared__ float2 win[FWD_WIN_MAX];
+  __shared__ float2 win[fwd_win<KM>()];
   int l, u;
   if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
   if (!g.tiled || u >= g.nfitems) return;
   float2* __restrict__ go = gx_old + row_off[l];
   float2* __restrict__ rr = r2 + row_off[l];
-  g_forward_tile(
+  g_forward_tile<KM>(
       g, u, Fx + (size_t)l * No, win, [=](int r) { return go[r]; },
       [=](int r, float2 gx, float2 old) {
         rr[r] = make_float2(2.f * gx.x - old.x, 2.f * gx.y - old.y);
@@ -682,19 +687,20 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_forward_tiled(const GDev* 
       });
 }
 
+template <int KM>
 __global__ __launch_bounds__(ADJ_THREADS) void k_data_apply_tiled(const GDev* __restrict__ gd,
                                                                   const int64_t* __restrict__ row_off,
                                                                   const float2* __restrict__ Fx, int64_t No,
                                                                   const float* __restrict__ lam,
                                                                   float2* __restrict__ y, int L, int units) {
-  __shared__ float2 win[FWD_WIN_MAX];
+  __shared__ float2 win[fwd_win<KM>()];
   int l, u;
   if (!xcd_map(L, units, l, u)) return;
   const GDev g = gd[l];
   if (!g.tiled || u >= g.nfitems) return;
   float2* __restrict__ yo = y + row_off[l];
   const float* __restrict__ lm = lam ? lam + row_off[l] : nullptr;
-  g_forward_tile(
+  g_forward_tile<KM>(
       g, u, Fx + (size_t)l * No, win, [=](int r) { return lm ? lm[r] : 1.f; },
       [=](int r, float2 v, float s) { yo[r] = lm ? cscale(v, s) : v; });
 }
@@ -702,8 +708,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_data_apply_tiled(const GDev* __
 // G Fx -> r2 = 2 G Fx - gx_old, gx_old <- G Fx (non-DR form) for all channels
 void launch_data_forward(fhs_ctx* ctx, const fhs_data* D, float2* gx_old, unsigned row_grid, int row_units) {
   if (D->fwd_grid > 0) {
-    k_data_forward_tiled<<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
-        D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L, D->fwd_grid);
+    if (D->kmax == 8)
+      k_data_forward_tiled<8><<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
+          D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L, D->fwd_grid);
+    else
+      k_data_forward_tiled<16><<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
+          D->gdev, D->d_row_off, D->Fx, D->No, gx_old, D->r2, D->L, D->fwd_grid);
     check_launch();
   }
   if (D->any_rows) {
@@ -828,8 +838,12 @@ void data_forward(fhs_ctx* ctx, fhs_data* D, const float* x, const float* lam, f
   Scope sc(ctx, "data_forward");
   const int units = (int)cdiv(D->maxM, 256);
   if (D->fwd_grid > 0) {
-    k_data_apply_tiled<<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
-        D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L, D->fwd_grid);
+    if (D->kmax == 8)
+      k_data_apply_tiled<8><<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
+          D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L, D->fwd_grid);
+    else
+      k_data_apply_tiled<16><<<xcd_grid(D->L, D->fwd_grid), ADJ_THREADS, 0, ctx->stream>>>(
+          D->gdev, D->d_row_off, D->Fx, D->No, lam, y, D->L, D->fwd_grid);
     check_launch();
   }
   if (D->any_rows)
@@ -929,6 +943,9 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
       D->row_off[l + 1] = D->row_off[l] + G[l]->M;
       D->maxM = std::max<int>(D->maxM, (int)G[l]->M);
       if (G[l]->d.tiled) {
+        FHS_CHECK(D->kmax == 0 || D->kmax == G[l]->d.kmax,
+                  "data_create: the channels' separable G must share their row width (8 or 16 slots)");
+        D->kmax = G[l]->d.kmax;
         D->tile_grid = std::max<int>(D->tile_grid, G[l]->d.nunits);
         D->red_grid = std::max<int>(D->red_grid, G[l]->d.nred);
         D->tile_lds = std::max(D->tile_lds, tile_lds_bytes(G[l]->d.Ky, G[l]->d.Kx));

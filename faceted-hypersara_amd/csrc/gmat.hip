@@ -172,11 +172,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint_reduce(GDev g, flo
   g_adjoint_tile_reduce(g, blockIdx.x, out);
 }
 
+template <int KM>
 __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_forward_tiled(GDev g, const float2* __restrict__ Fx,
                                                                   const float* __restrict__ lam,
                                                                   float2* __restrict__ y) {
-  __shared__ float2 win[FWD_WIN_MAX];
-  g_forward_tile(
+  __shared__ float2 win[fwd_win<KM>()];
+  g_forward_tile<KM>(
       g, blockIdx.x, Fx, win, [&](int r) { return lam ? lam[r] : 1.f; },
       [&](int r, float2 v, float s) { y[r] = lam ? cscale(v, s) : v; });
 }
@@ -190,6 +191,7 @@ __global__ void k_gmat_forward(GDev g, const float2* __restrict__ Fx,
   y[m] = acc;
 }
 
+template <int KM>
 __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint(GDev g, const float2* __restrict__ y,
                                                                const float* __restrict__ lam,
                                                                float2* __restrict__ out) {
@@ -198,8 +200,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_gmat_adjoint(GDev g, const floa
   auto wl = [&](int r) { return cscale(y[r], lam[r]); };
   auto wp = [&](int r) { return y[r]; };
   if (g.tiled) {
-    if (lam) g_adjoint_tile(g, blockIdx.x, wl, out, lds);
-    else g_adjoint_tile(g, blockIdx.x, wp, out, lds);
+    if (lam) g_adjoint_tile<KM>(g, blockIdx.x, wl, out, lds);
+    else g_adjoint_tile<KM>(g, blockIdx.x, wp, out, lds);
   } else {
     float2* prod = reinterpret_cast<float2*>(lds);
     if (lam) g_adjoint_unit(g, blockIdx.x, wl, none, out, prod);
@@ -356,25 +358,28 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
   *out = nullptr;
   return guard(ctx, [&] {
     FHS_CHECK(M > 0 && M < (1ll << 26), "gmat_create_separable: M must be in [1, 2^26)");
-    FHS_CHECK(Ky >= 1 && Kx >= 1 && Ky <= SEP_KMAX && Kx <= SEP_KMAX && Ky * Kx <= 64 && Ky <= Kty &&
-                  Kx <= Ktx,
-              "gmat_create_separable: 1 <= Ky, Kx <= 8");
+    FHS_CHECK(Ky >= 1 && Kx >= 1 && Ky <= SEP_KMAX && Kx <= SEP_KMAX && Ky <= Kty && Kx <= Ktx,
+              "gmat_create_separable: 1 <= Ky, Kx <= 16");
     const int64_t ncols = (int64_t)Kty * Ktx;
     FHS_CHECK(ncols < (1ll << 31), "gmat_create_separable: grid too large");
     const int K = Ky * Kx;
     const int64_t nnz = M * K;
     FHS_CHECK(nnz < (1ll << 31), "gmat_create_separable: nnz out of range");
-    std::vector<float2> sep((size_t)SEP_ROW * M, make_float2(0.f, 0.f));
+    const int km = sep_kmax_for(Ky, Kx), row = 2 * km;
+    std::vector<float2> sep((size_t)row * M, make_float2(0.f, 0.f));
     std::vector<int> anc(anchor, anchor + M);
     for (int64_t m = 0; m < M; ++m) {
       const int a = anchor[m];
       FHS_CHECK(a >= 0 && a < ncols, "gmat_create_separable: anchor out of range");
       for (int jy = 0; jy < Ky; ++jy)
-        sep[(size_t)m * SEP_ROW + jy] = make_float2(cy[2 * (m * Ky + jy)], cy[2 * (m * Ky + jy) + 1]);
+        sep[(size_t)m * row + jy] = make_float2(cy[2 * (m * Ky + jy)], cy[2 * (m * Ky + jy) + 1]);
       for (int jx = 0; jx < Kx; ++jx)
-        sep[(size_t)m * SEP_ROW + SEP_KMAX + jx] = make_float2(cx[2 * (m * Kx + jx)], cx[2 * (m * Kx + jx) + 1]);
+        sep[(size_t)m * row + km + jx] = make_float2(cx[2 * (m * Kx + jx)], cx[2 * (m * Kx + jx) + 1]);
     }
     const bool tiled = tile_grid_ok(Kty, Ktx, Ky, Kx);
+    // the CSC of small grids packs (row << 6) | tap: 64 taps at most
+    FHS_CHECK(tiled || K <= 64, "gmat_create_separable: more than 64 taps per row needs a grid of whole 32 x 32 "
+                                "tiles (the tiled adjoint)");
     std::vector<int> colptr, units;
     std::vector<unsigned> pk;
     TileLists tl;
@@ -394,6 +399,7 @@ int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, in
     g->mem[1] = upload(sep);
     GDev& d = g->d;
     d.layout = G_SEP;
+    d.kmax = km;
     d.M = (int)M;
     d.K = K;
     d.Kty = Kty;
@@ -464,8 +470,9 @@ int fhs_apply_direct_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* Fx,
     Scope sc(ctx, "gmat_forward");
     if (G->d.tiled) {
       if (G->d.nfitems > 0)
-        k_gmat_forward_tiled<<<G->d.nfitems, ADJ_THREADS, 0, ctx->stream>>>(G->d, (const float2*)Fx, Lambda,
-                                                                           (float2*)y);
+        (G->d.kmax > 8 ? k_gmat_forward_tiled<16> : k_gmat_forward_tiled<8>)<<<G->d.nfitems, ADJ_THREADS, 0,
+                                                                               ctx->stream>>>(
+            G->d, (const float2*)Fx, Lambda, (float2*)y);
     } else {
       k_gmat_forward<<<cdiv(G->M, 256), 256, 0, ctx->stream>>>(G->d, (const float2*)Fx, Lambda, (float2*)y);
     }
@@ -480,7 +487,9 @@ int fhs_apply_adjoint_operator(fhs_ctx* ctx, const fhs_gmat* G, const void* y,
     FHS_CHECK(G && Fx && y, "apply_adjoint_operator: null argument");
     Scope sc(ctx, "gmat_adjoint");
     const size_t lds = G->d.tiled ? tile_lds_bytes(G->d.Ky, G->d.Kx) : ADJ_CSC_LDS_BYTES;
-    k_gmat_adjoint<<<G->d.nunits, ADJ_THREADS, lds, ctx->stream>>>(G->d, (const float2*)y, Lambda, (float2*)Fx);
+    auto kern = G->d.kmax > 8 ? k_gmat_adjoint<16> : k_gmat_adjoint<8>;
+    allow_dynamic_lds((const void*)kern, lds);
+    kern<<<G->d.nunits, ADJ_THREADS, lds, ctx->stream>>>(G->d, (const float2*)y, Lambda, (float2*)Fx);
     check_launch();
     if (G->d.tiled && G->d.nred > 0) {
       k_gmat_adjoint_reduce<<<G->d.nred, ADJ_THREADS, 0, ctx->stream>>>(G->d, (float2*)Fx);

@@ -26,8 +26,11 @@
 namespace fhs {
 
 enum GLayout { G_ELL = 0, G_WINDOW = 1, G_SEP = 2 };
-constexpr int SEP_KMAX = 8;   // max taps per axis of the separable layout
-constexpr int SEP_ROW = 2 * SEP_KMAX;   // complex slots per separable row (128 B)
+constexpr int SEP_KMAX = 16;  // max taps per axis of the separable layout
+// A separable row holds 2 kmax complex slots (cy in 0..kmax-1, cx in
+// kmax..2 kmax-1, zero padded), kmax = 8 (128-byte rows, windows of up to
+// 8 taps: the min-max interpolator) or 16 (256-byte rows: w-projected rows).
+inline int sep_kmax_for(int Ky, int Kx) { return (Ky <= 8 && Kx <= 8) ? 8 : 16; }
 
 struct GDev {
   int layout;
@@ -36,7 +39,8 @@ struct GDev {
   int ncols;
   const int* cols;      // ELL: [K][M]
   const int* anchor;    // WINDOW: [M]
-  const float2* vals;   // ELL / WINDOW: [K][M]; SEP: [M][SEP_ROW]
+  const float2* vals;   // ELL / WINDOW: [K][M]; SEP: [M][2 kmax]
+  int kmax;             // SEP: complex slots per axis in a row (8 or 16)
   const int* colptr;    // [ncols + 1]
   const int* csc_rows;  // [nnz]
   const float2* csc_vals;
@@ -70,12 +74,13 @@ struct GDev {
   int nact;
 };
 
-// the 16 complex slots of separable row m (8 x 16-byte loads)
+// the 2 KM complex slots of separable row m (KM 16-byte loads); KM = g.kmax
+template <int KM>
 __device__ __forceinline__ void sep_load(const GDev& g, int m, float2* cy, float2* cx) {
-  const float4* p = reinterpret_cast<const float4*>(g.vals + (size_t)m * SEP_ROW);
+  const float4* p = reinterpret_cast<const float4*>(g.vals + (size_t)m * (2 * KM));
 #pragma unroll
-  for (int i = 0; i < SEP_KMAX / 2; ++i) {
-    const float4 a = p[i], b = p[SEP_KMAX / 2 + i];
+  for (int i = 0; i < KM / 2; ++i) {
+    const float4 a = p[i], b = p[KM / 2 + i];
     cy[2 * i] = make_float2(a.x, a.y);
     cy[2 * i + 1] = make_float2(a.z, a.w);
     cx[2 * i] = make_float2(b.x, b.y);
@@ -89,7 +94,7 @@ __device__ __forceinline__ void g_csc_entry(const GDev& g, int e, float2& v, int
     const unsigned pk = g.csc_pk[e];
     r = (int)(pk >> 6);
     const int tap = (int)(pk & 63u), jx = tap / g.Ky, jy = tap - jx * g.Ky;
-    v = cmul(g.vals[(size_t)r * SEP_ROW + jy], g.vals[(size_t)r * SEP_ROW + SEP_KMAX + jx]);
+    v = cmul(g.vals[(size_t)r * 2 * g.kmax + jy], g.vals[(size_t)r * 2 * g.kmax + g.kmax + jx]);
   } else {
     v = g.csc_vals[e];
     r = g.csc_rows[e];
@@ -116,20 +121,40 @@ __device__ __forceinline__ float2 g_row_apply(const GDev& g, int m, F&& f) {
       acc.x += v.x * x.x - v.y * x.y;
       acc.y += v.x * x.y + v.y * x.x;
     }
+  } else if (g.layout == G_SEP && g.kmax > 8) {
+    const int a = g.anchor[m];
+    const int ky0 = a % g.Kty, kx0 = a / g.Kty;
+    const float2* row = g.vals + (size_t)m * 2 * g.kmax;
+    for (int jx = 0; jx < g.Kx; ++jx) {
+      int kx = kx0 + jx;
+      if (kx >= g.Ktx) kx -= g.Ktx;
+      const int colbase = kx * g.Kty;
+      float2 s = make_float2(0.f, 0.f);
+      for (int jy = 0; jy < g.Ky; ++jy) {
+        int ky = ky0 + jy;
+        if (ky >= g.Kty) ky -= g.Kty;
+        const float2 x = f(colbase + ky), c = row[jy];
+        s.x += c.x * x.x - c.y * x.y;
+        s.y += c.x * x.y + c.y * x.x;
+      }
+      const float2 c = row[g.kmax + jx];
+      acc.x += c.x * s.x - c.y * s.y;
+      acc.y += c.x * s.y + c.y * s.x;
+    }
   } else if (g.layout == G_SEP) {
     const int a = g.anchor[m];
     const int ky0 = a % g.Kty, kx0 = a / g.Kty;
-    float2 cy[SEP_KMAX], cx[SEP_KMAX];
-    sep_load(g, m, cy, cx);
+    float2 cy[8], cx[8];
+    sep_load<8>(g, m, cy, cx);
 #pragma unroll
-    for (int jx = 0; jx < SEP_KMAX; ++jx) {   // fully unrolled: cy / cx stay in registers
+    for (int jx = 0; jx < 8; ++jx) {   // fully unrolled: cy / cx stay in registers
       if (jx < g.Kx) {
         int kx = kx0 + jx;
         if (kx >= g.Ktx) kx -= g.Ktx;
         const int colbase = kx * g.Kty;
         float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int jy = 0; jy < SEP_KMAX; ++jy) {
+        for (int jy = 0; jy < 8; ++jy) {
           if (jy < g.Ky) {
             int ky = ky0 + jy;
             if (ky >= g.Kty) ky -= g.Kty;
@@ -235,11 +260,13 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
 constexpr int TILE_Y = 32, TILE_X = 32;
 constexpr int TILE_CH = ADJ_THREADS;                 // rows staged per chunk
 constexpr int TILE_PTS = TILE_Y * TILE_X / ADJ_THREADS;
-constexpr int TILE_CELLS_MAX = (TILE_Y + SEP_KMAX - 1) * (TILE_X + SEP_KMAX - 1);
+template <int KM>
+constexpr int tile_cells_max() { return (TILE_Y + KM - 1) * (TILE_X + KM - 1); }
 constexpr int TILE_ITEM = 4 * TILE_CH;               // listed rows per work item
 constexpr int TILE_EMPTY_GROUP = 8;                  // empty tiles zeroed per work item
 constexpr int FWD_ITEM = 1024;                       // rows per forward work item
-constexpr int FWD_WIN_MAX = (TILE_Y + SEP_KMAX - 1) * (TILE_X + SEP_KMAX - 1);
+template <int KM>
+constexpr int fwd_win() { return (TILE_Y + KM - 1) * (TILE_X + KM - 1); }   // forward window (complex)
 
 // LDS of a tile workgroup (dynamic shared memory): the staged rows with an
 // odd stride of Kx + Ky + 1 complex values (conj(cx) w, then conj(cy)), so
@@ -276,8 +303,9 @@ inline bool tile_grid_ok(int Kty, int Ktx, int Ky, int Kx) {
 // a[c] = min(a[c], a[c+1], ..., a[n-1]) for c < n, by the ADJ_THREADS
 // threads of the workgroup (each owns a contiguous segment); wsum: LDS
 // scratch of ADJ_THREADS / 64 ints.  Ends with a barrier.
+template <int KM>
 __device__ __forceinline__ void tile_suffix_min(int* a, int n, int* wsum) {
-  constexpr int PER = (TILE_CELLS_MAX + 1 + ADJ_THREADS - 1) / ADJ_THREADS;
+  constexpr int PER = (tile_cells_max<KM>() + 1 + ADJ_THREADS - 1) / ADJ_THREADS;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   int v[PER];
   int m = 0x7fffffff;
@@ -307,7 +335,7 @@ __device__ __forceinline__ void tile_suffix_min(int* a, int n, int* wsum) {
   __syncthreads();
 }
 
-template <class W>
+template <int KM, class W>
 __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, float2* __restrict__ out,
                                                unsigned char* lds) {
   const TileView s = tile_view(g, lds);
@@ -348,11 +376,11 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
     }
     if (i < n) {
       const float2 v = w(r);
-      float2 cy[SEP_KMAX], cx[SEP_KMAX];
-      sep_load(g, r, cy, cx);
+      float2 cy[KM], cx[KM];
+      sep_load<KM>(g, r, cy, cx);
       float2* q = s.qc + (size_t)i * s.stride;
 #pragma unroll
-      for (int j = 0; j < SEP_KMAX; ++j) {
+      for (int j = 0; j < KM; ++j) {
         if (j < g.Kx) {
           const float2 c = cx[j];
           q[j] = make_float2(c.x * v.x + c.y * v.y, c.x * v.y - c.y * v.x);   // conj(c) v
@@ -367,7 +395,7 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
     // occupied cell marks it, then a block-wide suffix minimum fills the rest
     if (i < n && (i == 0 || s.cell[i - 1] != s.cell[i])) s.start[s.cell[i]] = i;
     __syncthreads();
-    tile_suffix_min(s.start, ncells + 1, s.cell + TILE_CH);
+    tile_suffix_min<KM>(s.start, ncells + 1, s.cell + TILE_CH);
     // occupied cell columns of this chunk (cells are sorted, x-major)
     const int cxmin = s.cell[0] / ncy, cxmax = s.cell[n - 1] / ncy;
 #pragma unroll
@@ -422,14 +450,14 @@ __device__ __forceinline__ void g_adjoint_tile_reduce(const GDev& g, int r, floa
 // before the stores), then every row of the item reads its Ky x Kx taps from
 // LDS instead of gathering them through the caches.  emit(r, G Fx) stores
 // row r's value.
-template <class P, class E>
+template <int KM, class P, class E>
 __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const float2* __restrict__ Fx,
                                                float2* win, P&& pre, E&& emit) {
   const int4 it = g.fitems[item];
   const int t = it.x;
   const int ty0 = (t % g.tiles_y) * TILE_Y, tx0 = (t / g.tiles_y) * TILE_X;
   const int wy = TILE_Y + g.Ky - 1, wx = TILE_X + g.Kx - 1, nw = wy * wx;
-  constexpr int PER = (FWD_WIN_MAX + ADJ_THREADS - 1) / ADJ_THREADS;
+  constexpr int PER = (fwd_win<KM>() + ADJ_THREADS - 1) / ADJ_THREADS;
   float2 v[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
@@ -456,16 +484,16 @@ __device__ __forceinline__ void g_forward_tile(const GDev& g, int item, const fl
     const auto pf = pre(r);   // the caller's per-row loads, issued before the arithmetic
     const int a = g.anchor[r];
     const int ly = a % g.Kty - ty0, lx = a / g.Kty - tx0;
-    float2 cy[SEP_KMAX], cx[SEP_KMAX];
-    sep_load(g, r, cy, cx);
+    float2 cy[KM], cx[KM];
+    sep_load<KM>(g, r, cy, cx);
     float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int jx = 0; jx < SEP_KMAX; ++jx) {
+    for (int jx = 0; jx < KM; ++jx) {
       if (jx < g.Kx) {
         const float2* col = win + (lx + jx) * wy + ly;
         float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int jy = 0; jy < SEP_KMAX; ++jy) {
+        for (int jy = 0; jy < KM; ++jy) {
           if (jy < g.Ky) {
             const float2 x = col[jy];
             s.x += cy[jy].x * x.x - cy[jy].y * x.y;

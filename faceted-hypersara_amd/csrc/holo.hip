@@ -66,13 +66,13 @@ __device__ __forceinline__ void holo_point(const GDev& g, const HoloGeom& h, int
     const unsigned pk = g.csc_pk[e];
     const int m = (int)(pk >> 6), t = (int)(pk & 63u);
     const int jx = t / g.Ky, jy = t - jx * g.Ky;
-    const float2* row = g.vals + (size_t)m * SEP_ROW;
-    const float2 gp = cmul(row[jy], row[SEP_KMAX + jx]);
+    const float2* row = g.vals + (size_t)m * 2 * g.kmax;
+    const float2 gp = cmul(row[jy], row[g.kmax + jx]);
 #pragma unroll
     for (int s = 0; s < HOLO_SLOTS; ++s) {
       const int ty = jy + dy[s], tx = jx + dx[s];
       if (q[s] >= 0 && ty >= 0 && ty < g.Ky && tx >= 0 && tx < g.Kx) {
-        const float2 gq = cmul(row[ty], row[SEP_KMAX + tx]);
+        const float2 gq = cmul(row[ty], row[g.kmax + tx]);
         ar[s] += (double)gp.x * gq.x + (double)gp.y * gq.y;     // conj(gp) * gq
         ai[s] += (double)gp.x * gq.y - (double)gp.y * gq.x;
       }
@@ -198,6 +198,7 @@ int fhs_gmat_create_holographic(fhs_ctx* ctx, const fhs_gmat* G, fhs_gmat** out)
   return guard(ctx, [&] {
     FHS_CHECK(G, "gmat_create_holographic: null G");
     FHS_CHECK(G->d.layout == G_SEP, "gmat_create_holographic: G must have separable (min-max) rows");
+    FHS_CHECK(G->d.Ky * G->d.Kx <= 64, "gmat_create_holographic: at most 64 taps per row (8 x 8 windows)");
     FHS_CHECK(G->device == ctx->device, "gmat_create_holographic: G lives on another device");
     const GDev& gd = G->d;
     HoloGeom h{gd.Kty, gd.Ktx, gd.Ky, gd.Kx, 2 * gd.Ky - 1, 2 * gd.Kx - 1, (int64_t)gd.Kty * gd.Ktx};

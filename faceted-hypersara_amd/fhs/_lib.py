@@ -55,6 +55,9 @@ PROTOTYPES = {
     'fhs_nufft_interp_coefficients': [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p, c_int, c_int,
                                       c_int, c_int, c_double_p, c_double_p, c_void_p, c_void_p, c_void_p, c_i64,
                                       c_void_p, c_void_p, c_void_p],
+    'fhs_nufft_wproj_rows': [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double_p,
+                             c_double_p, c_double, c_double, c_void_p, c_i64, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p],
     'fhs_nufft_destroy': [c_void_p],
     'fhs_apply_scaled_fourier_transform': [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_apply_scaled_fourier_adjoint': [c_void_p, c_void_p, c_void_p, c_int, c_float_p, c_void_p],

@@ -135,7 +135,9 @@ class GMatrix:
             ctx.h, M, Kty, Ktx, Ky, Kx, anchor.ctypes.data_as(C.POINTER(C.c_int32)),
             cyf.ctypes.data_as(C.POINTER(C.c_float)), cxf.ctypes.data_as(C.POINTER(C.c_float)), C.byref(h)),
             'gmat_create_separable')
-        return cls(ctx, h, M, Kty * Ktx, M * Ky * Kx)
+        g = cls(ctx, h, M, Kty * Ktx, M * Ky * Kx)
+        g.Ky, g.Kx = Ky, Kx
+        return g
 
     def __del__(self):
         h = getattr(self, 'h', None)

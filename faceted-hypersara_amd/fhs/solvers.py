@@ -57,8 +57,12 @@ class Measurement:
         self.nufft = Nufft2D(P.Ny, P.Nx)
         self.plan = self.nufft.plan(ctx)
         Kty, Ktx = self.nufft.Kty, self.nufft.Ktx
-        # interpolation coefficients, computed once per channel on the GPU (anchor, cy, cx)
-        coef = [self.nufft.separable_gpu(ctx, P.u[l], P.v[l]) for l in range(P.L)]
+        # interpolation coefficients, computed once per channel on the GPU
+        # (anchor, cy, cx): min-max rows, or w-projected rows with the w-term
+        if getattr(P, 'w', None):
+            coef = [self.nufft.separable_wproj_gpu(ctx, P.u[l], P.v[l], P.w[l], P.fov) for l in range(P.L)]
+        else:
+            coef = [self.nufft.separable_gpu(ctx, P.u[l], P.v[l]) for l in range(P.L)]
         if not P.y:
             Graw = [GMatrix.from_separable(ctx, a, cy, cx, Kty, Ktx) for a, cy, cx in coef]
             Draw = DataOperator(ctx, self.plan, Graw, P.blocks)

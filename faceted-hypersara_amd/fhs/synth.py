@@ -71,8 +71,8 @@ def sky_image(rng, Ny, Nx, n_sources=200, grf_weight=0.3):
     return img / img.max()
 
 
-def spectral_cube(rng, x0, L, dnu=0.05):
-    nu = 1.0 + dnu * np.arange(L)
+def spectral_cube(rng, x0, L, dnu=0.05, nu=None):
+    nu = 1.0 + dnu * np.arange(L) if nu is None else np.asarray(nu, dtype=np.float64)
     alpha = rng.normal(-0.7, 0.2, x0.shape)
     beta = rng.normal(0.0, 0.05, x0.shape)
     ln = np.log(nu)
@@ -92,6 +92,8 @@ class Problem:
     nu: np.ndarray
     seed: int
     input_snr: float
+    w: list = field(default_factory=list)          # per channel w (wavelengths), empty: no w-term
+    fov: float = 0.0                               # field of view (direction cosines) of the w-term
     y: list = field(default_factory=list)          # whitened visibilities per channel
     sigma_noise: list = field(default_factory=list)
     epsilon: list = field(default_factory=list)    # per channel: list of block bounds
@@ -102,17 +104,26 @@ class Problem:
 
 
 def make_problem(Ny, Nx, L, M_base, nblocks, seed=0, flag_range=(0.1, 0.3), input_snr=40.0,
-                 dnu=0.05, n_sources=200):
+                 dnu=0.05, n_sources=200, w_max=None, fov=0.08, nu=None, nu_max=None):
+    """Seeded problem.  ``w_max``: base w uniform in [0, w_max] wavelengths
+    (scaled per channel with u and v) and field of view ``fov`` for the
+    w-term; ``nu``: the channels' frequencies (default 1 + dnu l) and
+    ``nu_max`` the frequency the uv coverage is scaled against (default
+    max(nu)) -- one spectral sub-cube of a larger cube takes its share of
+    the cube's frequencies and the cube's nu_max, and the same seed gives
+    every sub-cube the same sky, spectral indices and base coverage."""
     rng = np.random.default_rng(seed)
     x0 = sky_image(rng, Ny, Nx, n_sources=n_sources)
     if L == 1:
-        X0, nu = x0[:, :, None], np.ones(1)
+        X0, nu_ = x0[:, :, None], np.ones(1)
     else:
-        X0, nu = spectral_cube(rng, x0, L, dnu)
+        X0, nu_ = spectral_cube(rng, x0, L, dnu, nu)
+    nu = nu_
     ub, vb = gaussian_uv(rng, M_base)
-    us, vs, blocks = [], [], []
+    wb = rng.uniform(0.0, w_max, M_base) if w_max else None
+    us, vs, ws, blocks = [], [], [], []
     for l in range(L):
-        scale = nu[l] / nu.max()
+        scale = nu[l] / (nu.max() if nu_max is None else nu_max)
         if L > 1 and flag_range is not None:
             frac = rng.uniform(*flag_range)
             keep = np.sort(rng.permutation(M_base)[:int(round(M_base * (1 - frac)))])
@@ -120,9 +131,11 @@ def make_problem(Ny, Nx, L, M_base, nblocks, seed=0, flag_range=(0.1, 0.3), inpu
             keep = np.arange(M_base)
         us.append(ub[keep] * scale)
         vs.append(vb[keep] * scale)
+        if wb is not None:
+            ws.append(wb[keep] * scale)
         b = np.linspace(0, len(keep), nblocks + 1).astype(np.int64)
         blocks.append([int(b[i + 1] - b[i]) for i in range(nblocks)])
-    return Problem(Ny, Nx, L, X0, us, vs, blocks, nu, seed, input_snr)
+    return Problem(Ny, Nx, L, X0, us, vs, blocks, nu, seed, input_snr, w=ws, fov=fov if ws else 0.0)
 
 
 def host_visibilities(problem):
@@ -131,6 +144,8 @@ def host_visibilities(problem):
     FFT), so a problem's data do not depend on the device that builds it."""
     from .nufft import Nufft2D
     P = problem
+    if getattr(P, 'w', None):
+        raise NotImplementedError('host_visibilities: no w-term on the host path (use the GPU data path)')
     nf = Nufft2D(P.Ny, P.Nx)
     Kty, Ktx = nf.Kty, nf.Ktx
     sn = np.outer(nf.ay.s, nf.ax.s)

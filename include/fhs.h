@@ -81,8 +81,9 @@ int fhs_gmat_create_window_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx,
  * cy[m*Ky + jy] * cx[m*Kx + jx] on the wrapped window anchored at
  * anchor[m] = ky0 + Kty*kx0 (interleaved complex, host pointers; any
  * per-row scaling such as the noise whitening folded into cx).
- * Ky, Kx <= 8, M < 2^26.  Same products as the window form, 3.5x less
- * coefficient traffic; the adjoint CSC stores 4 B per nonzero. */
+ * Ky, Kx <= 16 (rows of 8 or 16 slots per axis), M < 2^26; more than 64
+ * taps per row (w-projected rows) needs a grid of whole 32 x 32 tiles.
+ * Same products as the window form, 3.5x less coefficient traffic. */
 int fhs_gmat_create_separable_host(fhs_ctx* ctx, int64_t M, int Kty, int Ktx, int Ky, int Kx,
                                    const int32_t* anchor, const float* cy, const float* cx,
                                    fhs_gmat** out);
@@ -140,6 +141,24 @@ int fhs_nufft_interp_coefficients(fhs_ctx* ctx, int Ny, int Kty, int Jy, int nay
                                   const double* Ty, int Nx, int Ktx, int Jx, int nax, const double* ax,
                                   const double* Tx, const double* u, const double* v,
                                   const double* row_scale, int64_t M, int32_t* anchor, float* cy, float* cx);
+/* W-projected rows of G from min-max rows (anchor_in, cy_in [M][Jy],
+ * cx_in [M][Jx] of fhs_nufft_interp_coefficients, device): the per-
+ * visibility step of op_p_nufft_wproj_dde
+ * (lib/operators/util_load_data_gen_measurement_operator.m:119-121) with
+ * the Fresnel w-phase exp(i pi w (l^2 + m^2)), which keeps rows separable:
+ * per axis cy' = (1/Kty) cy correlated with the w-kernel Chat(w)[d], |d| <= S,
+ * then cut to the largest taps holding keep_y (keep_x) of the row's energy
+ * (the reference's GEnergyL2 split over the two axes, keep = sqrt(GEnergyL2)).
+ * table_y / table_x: host FP64 complex [nbins][2S + 1], Chat at w = b dw
+ * (b = 0..nbins-1, |w| <= (nbins - 1) dw; CEnergyL2 applied by the caller);
+ * w: device double[M] in wavelengths.  Outputs (device): anchor_out
+ * (S points before the input anchor on each axis), cy_out [M][Jy + 2 Sy],
+ * cx_out [M][Jx + 2 Sx] (Jy + 2 Sy, Jx + 2 Sx <= 16): the layout of
+ * fhs_gmat_create_separable_host. */
+int fhs_nufft_wproj_rows(fhs_ctx* ctx, int Kty, int Ktx, int Jy, int Jx, int Sy, int Sx, int nbins, double dw,
+                         const double* table_y, const double* table_x, double keep_y, double keep_x,
+                         const double* w, int64_t M, const int32_t* anchor_in, const float* cy_in,
+                         const float* cx_in, int32_t* anchor_out, float* cy_out, float* cx_out);
 /* Fx(:, l) = A(x(:, :, l)), l = 1..L
  * src/apply_scaled_fourier_transform.m:21-24.  x: float[L][Nx][Ny];
  * Fx: complex[L][Kty*Ktx]. */

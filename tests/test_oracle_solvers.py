@@ -152,6 +152,7 @@ def _kkt(kind, x, info, p, op, geom):
     out['l21_fenchel'] = (den - np.sum(v1 * R)) / den if den > 0 else 0.0
     grad = np.stack([sara.psi(v1[:, l]) for l in range(L)], axis=2)
     scale = np.abs(grad).max()
+    scale = scale if scale > 0 else 1.0      # a wrecked iterate (planted mistakes) can zero v1
     # low-rank prior: v0_q shares the singular vectors of X_q = w_q . x_q,
     # equal to the threshold on its active singular values, below it elsewhere
     nuc = []
