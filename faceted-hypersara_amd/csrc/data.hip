@@ -815,9 +815,9 @@ void data_set_epsilon(fhs_ctx* ctx, fhs_data* D, const double* h_eps) {
   FHS_HIP(hipMemcpyAsync(D->d_eps, h_eps, D->nblocks * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
 }
 
-void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int dr, const float* lam) {
-  // gx_old = G A(x): run the forward with a zero previous state, keep gx
-  nufft_forward(ctx, D->A, x, D->L, D->Fx);
+// gx_old = G Fx for the grid vectors already in D->Fx: the forward rows
+// with a zero previous state, keeping gx
+static void data_gx_from_grid(fhs_ctx* ctx, fhs_data* D, float2* gx_old, int dr, const float* lam) {
   FHS_HIP(hipMemsetAsync(gx_old, 0, D->Mtot * sizeof(float2), ctx->stream));
   const int units = (int)cdiv(D->maxM, 256);
   const unsigned grid = xcd_grid(D->L, units);
@@ -830,6 +830,11 @@ void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int
   else
     launch_data_forward(ctx, D, gx_old, grid, units);
   check_launch();
+}
+
+void data_init_gx(fhs_ctx* ctx, fhs_data* D, const float* x, float2* gx_old, int dr, const float* lam) {
+  nufft_forward(ctx, D->A, x, D->L, D->Fx);
+  data_gx_from_grid(ctx, D, gx_old, dr, lam);
 }
 
 // y = Lambda .* G A x (all channels)
@@ -1063,6 +1068,35 @@ int fhs_data_init_fx_old(fhs_ctx* ctx, fhs_data* D, const float* x, const float*
     FHS_CHECK(D && x && Gx_old, "data_init_fx_old: bad argument");
     FHS_CHECK(!flag_dimensionality_reduction || D->square, "data_init_fx_old: DR needs square G");
     data_init_gx(ctx, D, x, (float2*)Gx_old, flag_dimensionality_reduction, Lambda);
+  });
+}
+
+int fhs_data_info(const fhs_data* D, int* Ny, int* Nx, int* L, int64_t* No, int64_t* Mtot, int* nblocks) {
+  if (!D) return FHS_ERR_ARG;
+  if (Ny) *Ny = D->A->Ny;
+  if (Nx) *Nx = D->A->Nx;
+  if (L) *L = D->L;
+  if (No) *No = D->No;
+  if (Mtot) *Mtot = D->Mtot;
+  if (nblocks) *nblocks = D->nblocks;
+  return FHS_OK;
+}
+
+int fhs_data_plan(const fhs_data* D, const fhs_nufft** plan) {
+  if (!D || !plan) return FHS_ERR_ARG;
+  *plan = D->A;
+  return FHS_OK;
+}
+
+int fhs_data_fx_old_to_state(fhs_ctx* ctx, fhs_data* D, const void* Fx_old, const float* Lambda,
+                             int flag_dimensionality_reduction, void* Gx_old) {
+  if (!ctx) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(D && Fx_old && Gx_old, "data_fx_old_to_state: bad argument");
+    FHS_CHECK(!flag_dimensionality_reduction || D->square, "data_fx_old_to_state: DR needs square G");
+    FHS_HIP(hipMemcpyAsync(D->Fx, Fx_old, (size_t)D->L * D->No * sizeof(float2), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    data_gx_from_grid(ctx, D, (float2*)Gx_old, flag_dimensionality_reduction, Lambda);
   });
 }
 

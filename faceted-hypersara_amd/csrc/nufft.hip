@@ -514,6 +514,15 @@ int fhs_nufft_create_host(fhs_ctx* ctx, int Ny, int Nx, int Kty, int Ktx, const 
   });
 }
 
+int fhs_nufft_info(const fhs_nufft* p, int* Ny, int* Nx, int* Kty, int* Ktx) {
+  if (!p) return FHS_ERR_ARG;
+  if (Ny) *Ny = p->Ny;
+  if (Nx) *Nx = p->Nx;
+  if (Kty) *Kty = p->Kty;
+  if (Ktx) *Ktx = p->Ktx;
+  return FHS_OK;
+}
+
 int fhs_nufft_destroy(fhs_nufft* p) {
   if (!p) return FHS_OK;
   (void)hipSetDevice(p->device);

@@ -27,7 +27,7 @@ static void fhs_mex_at_exit(void) {
   g_fhs_ctx = NULL;
 }
 
-static fhs_ctx* fhs_mex_ctx(void) {
+static inline fhs_ctx* fhs_mex_ctx(void) {
   if (!g_fhs_ctx) {
     if (fhs_ctx_create(0, &g_fhs_ctx) != FHS_OK) mexErrMsgIdAndTxt("fhs:ctx", "cannot create the GPU context");
     mexAtExit(fhs_mex_at_exit);
@@ -36,12 +36,12 @@ static fhs_ctx* fhs_mex_ctx(void) {
   return g_fhs_ctx;
 }
 
-static void fhs_mex_check(int st, const char* what) {
+static inline void fhs_mex_check(int st, const char* what) {
   if (st != FHS_OK) mexErrMsgIdAndTxt("fhs:call", "%s: %s", what, fhs_last_error(g_fhs_ctx));
 }
 
 /* handle stored in a struct: s.fhs_handle (uint64), s.fhs_kind (char) */
-static void* fhs_mex_handle(const mxArray* s, const char* kind) {
+static inline void* fhs_mex_handle(const mxArray* s, const char* kind) {
   const mxArray* h;
   const mxArray* k;
   char buf[32];
@@ -54,7 +54,7 @@ static void* fhs_mex_handle(const mxArray* s, const char* kind) {
 }
 
 /* complex double column (MATLAB) -> device complex64 */
-static void* fhs_mex_to_device_complex(const mxArray* a, int64_t* n_out) {
+static inline void* fhs_mex_to_device_complex(const mxArray* a, int64_t* n_out) {
   const int64_t n = (int64_t)mxGetNumberOfElements(a);
   float* h = (float*)mxMalloc(2 * n * sizeof(float) + 8);
   void* d = NULL;
@@ -80,7 +80,7 @@ static void* fhs_mex_to_device_complex(const mxArray* a, int64_t* n_out) {
 }
 
 /* real double array -> device float32 (NULL for []) */
-static float* fhs_mex_to_device_real(const mxArray* a, int64_t* n_out) {
+static inline float* fhs_mex_to_device_real(const mxArray* a, int64_t* n_out) {
   const int64_t n = (int64_t)mxGetNumberOfElements(a);
   float* h;
   void* d = NULL;
@@ -98,7 +98,7 @@ static float* fhs_mex_to_device_real(const mxArray* a, int64_t* n_out) {
 }
 
 /* device complex64[n] -> new complex double column */
-static mxArray* fhs_mex_from_device_complex(const void* d, int64_t n) {
+static inline mxArray* fhs_mex_from_device_complex(const void* d, int64_t n) {
   mxArray* out = mxCreateDoubleMatrix((mwSize)n, 1, mxCOMPLEX);
   mxComplexDouble* z = mxGetComplexDoubles(out);
   float* h = (float*)mxMalloc(2 * n * sizeof(float) + 8);

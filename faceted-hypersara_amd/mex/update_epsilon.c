@@ -26,6 +26,7 @@ static void gather(const mxArray* a, double* out, int64_t* k) {
   } else {
     int64_t i, m = (int64_t)mxGetNumberOfElements(a);
     const double* r = mxGetDoubles(a);
+    if (!r && m > 0) mexErrMsgIdAndTxt("fhs:args", "update_epsilon: cells of real double scalars");
     for (i = 0; i < m; ++i) out[(*k)++] = r[i];
   }
 }
@@ -53,7 +54,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   double *eps, *tb_d, *res;
   int *tb, n_updated = 0;
   (void)nlhs;
-  (void)fhs_mex_ctx();   /* error text comes from the context */
+  /* host logic only: no GPU context */
   if (nrhs != 8) mexErrMsgIdAndTxt("fhs:args", "[epsilon, t_block] = update_epsilon(epsilon, t, t_block, norm_res, "
                                               "tol_in, tol_out, steps, change_percentage)");
   n = count_leaves(prhs[0]);
@@ -67,10 +68,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   k = 0; gather(prhs[2], tb_d, &k);
   k = 0; gather(prhs[3], res, &k);
   for (i = 0; i < n; ++i) tb[i] = (int)tb_d[i];
-  fhs_mex_check(fhs_update_epsilon(eps, (int)mxGetScalar(prhs[1]), tb, res, (int)n, mxGetScalar(prhs[4]),
-                                   mxGetScalar(prhs[5]), (int)mxGetScalar(prhs[6]), mxGetScalar(prhs[7]),
-                                   &n_updated),
-                "update_epsilon");
+  if (fhs_update_epsilon(eps, (int)mxGetScalar(prhs[1]), tb, res, (int)n, mxGetScalar(prhs[4]), mxGetScalar(prhs[5]),
+                         (int)mxGetScalar(prhs[6]), mxGetScalar(prhs[7]), &n_updated) != FHS_OK)
+    mexErrMsgIdAndTxt("fhs:args", "update_epsilon: invalid arguments");
   for (i = 0; i < n; ++i) tb_d[i] = (double)tb[i];
   k = 0; plhs[0] = scatter(prhs[0], eps, &k);
   k = 0; plhs[1] = scatter(prhs[2], tb_d, &k);

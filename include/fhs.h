@@ -128,6 +128,8 @@ int fhs_holographic_grid(fhs_ctx* ctx, const fhs_gmat* G, const fhs_gmat* H, con
 int fhs_nufft_create_host(fhs_ctx* ctx, int Ny, int Nx, int Kty, int Ktx,
                           const double* sy, const double* sx, fhs_nufft** out);
 int fhs_nufft_destroy(fhs_nufft* p);
+/* Sizes of a plan (any pointer may be NULL): image Ny x Nx, grid Kty x Ktx. */
+int fhs_nufft_info(const fhs_nufft* p, int* Ny, int* Nx, int* Kty, int* Ktx);
 /* Min-max interpolation coefficients of M visibilities, computed on the GPU
  * in FP64: the per-visibility loop of op_p_nufft that builds G
  * (lib/operators/util_gen_measurement_operator_synth.m:85).  Per axis: image
@@ -217,12 +219,24 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
                     int nblocks, const int64_t* block_start, const int* block_channel,
                     fhs_data** out);
 int fhs_data_destroy(fhs_data* data);
+/* Sizes of a data state (any pointer may be NULL): image Ny x Nx, L
+ * channels, No = Kty*Ktx grid points, Mtot stacked rows, data blocks. */
+int fhs_data_info(const fhs_data* data, int* Ny, int* Nx, int* L, int64_t* No, int64_t* Mtot, int* nblocks);
+/* The scaled-FFT plan A the state was created with. */
+int fhs_data_plan(const fhs_data* data, const fhs_nufft** plan);
 /* Gx_old = G A(x) (Lambda (G + G') A(x) for the DR form): the state that
  * replaces Fx_old = A(x) (src/hs/hyperSARA.m, Fxi_old = apply_scaled_
  * fourier_transform(xsol, A, No)); by linearity G(2 Fx - Fx_old) =
  * 2 G Fx - Gx_old, so M complex per channel are kept instead of No. */
 int fhs_data_init_fx_old(fhs_ctx* ctx, fhs_data* data, const float* x, const float* Lambda,
                          int flag_dimensionality_reduction, void* Gx_old);
+/* The same state from the reference's own Fx_old = A(x_old) array
+ * (src/fhs/facetHyperSARA.m:574-576, src/hs/hyperSARA.m:446, consumed at
+ * src/update_dual_data_fidelity.m:92,132): Fx_old complex[L][Kty*Ktx] on the
+ * device (MATLAB's [No, L] column-major array); Gx_old = G_l Fx_old(:, l)
+ * per channel (Lambda (G + G') Fx_old(:, l) for the DR form). */
+int fhs_data_fx_old_to_state(fhs_ctx* ctx, fhs_data* data, const void* Fx_old, const float* Lambda,
+                             int flag_dimensionality_reduction, void* Gx_old);
 /* y = Lambda .* (G_l A x_l) for every channel (stacked): the measurement
  * operator of HS_forward_operator_G.m / util_operator_norm.m:45 (Lambda
  * NULL = 1, or sqrt(aW) for the preconditioned operator). */
