@@ -99,7 +99,28 @@ def parse():
     return a
 
 
-def algorithmic_bytes(meas, sara_s):
+def lowrank_bytes(lowrank, L, N):
+    """Per-launch algorithmic bytes of the nuclear-norm step (Q facets of
+    P_q pixels, sum P = sum_q P_q): the Gram reads v0 and the windowed facet
+    of x-hat (8 L sum P) and the windows (4 sum P); the reduce + eigen-solve
+    read the per-chunk FP64 partial Grams (Q * chunks * 256 * 8) and write
+    the L x L mixing matrices; the apply reads v0, x-hat and the windows and
+    writes v0 (12 L sum P + 4 sum P); the overlap-add reads every facet's v0
+    and window once and writes g0 (4 L sum P + 4 sum P + 4 L N)."""
+    if lowrank is None:
+        return {}
+    P = [int(a) * int(b) for a, b in zip(lowrank.geom['py'], lowrank.geom['px'])]
+    sP, Q = sum(P), len(P)
+    chunks = -(-max(P) // 1024)
+    out = {'nuclear_gram': 8 * L * sP + 4 * sP,
+           'nuclear_eig': Q * chunks * 256 * 8 + Q * 256 * 8 * 2,
+           'nuclear_apply': 12 * L * sP + 4 * sP}
+    if not (Q == 1 and sP == N):
+        out['overlap_add'] = 4 * L * sP + 4 * sP + 4 * L * N
+    return out
+
+
+def algorithmic_bytes(meas, sara_s, lowrank=None):
     """Per-launch algorithmic HBM bytes of the kernels for the layouts in use
     (complex = 8 B, real FP32 = 4 B; every operand read or written once).
     G is the separable min-max layout: Ky + Kx coefficients and an anchor
@@ -133,14 +154,15 @@ def algorithmic_bytes(meas, sara_s):
         'l21_threshold': 8 * L * sara_s + 4 * L * N + 4 * sara_s,
         'primal': 4 * L * N * 6,
         'fb_dual': 44 * sum(M),
+        **lowrank_bytes(lowrank, L, N),
     }
 
 
-def algorithmic_bytes_dr(meas, sara_s):
+def algorithmic_bytes_dr(meas, sara_s, lowrank=None):
     """The DR form's per-launch algorithmic bytes: every active mode's full
     Hermitian stencil ((2Ky-1)(2Kx-1) complex values) read once per apply, the
     grid vectors once; FFT and prior kernels as in the standard form."""
-    out = algorithmic_bytes(meas.meas, sara_s)
+    out = algorithmic_bytes(meas.meas, sara_s, lowrank)
     nf = meas.meas.nufft
     No = nf.No
     ns = (2 * nf.Ky - 1) * (2 * nf.Kx - 1)
@@ -313,7 +335,7 @@ def cube_main(args):
     rep = ctx.timing_report()
     ctx.timing_reset()
     meas0 = subs[0][1]
-    alg = algorithmic_bytes(meas0, eng.sara.s)
+    alg = algorithmic_bytes(meas0, eng.sara.s, eng.lowrank)
     dominant = max((k for k in rep if k in alg), key=lambda k: rep[k][0])
     avg_ms = rep[dominant][0] / rep[dominant][1]
     achieved = alg[dominant] / (avg_ms * 1e-3) / 1e9
@@ -504,7 +526,8 @@ def main():
     eng.set_concurrent(True)
     rep = ctx.timing_report()
     ctx.timing_reset()
-    alg = algorithmic_bytes_dr(meas, eng.sara.s) if dr else algorithmic_bytes(meas, eng.sara.s)
+    alg = (algorithmic_bytes_dr(meas, eng.sara.s, eng.lowrank) if dr else
+           algorithmic_bytes(meas, eng.sara.s, eng.lowrank))
     per_iter_ms = {k: v[0] / n_timed for k, v in rep.items()}
     dominant = max((k for k in rep if k in alg), key=lambda k: rep[k][0])
     avg_ms = rep[dominant][0] / rep[dominant][1]

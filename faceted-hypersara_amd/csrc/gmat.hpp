@@ -242,40 +242,192 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
 // Tile-gridding adjoint of the separable layout.
 //
 // The grid is cut into TILE_Y x TILE_X tiles.  At creation every row is
-// listed under each tile its Ky x Kx window touches (1, 2 or 4 tiles), and
-// each tile's list is sorted by the row's anchor cell relative to the tile
-// ("local cell": (TILE_Y + Ky - 1) x (TILE_X + Kx - 1) cells, x-major).  One
-// workgroup owns one tile: it stages 256 listed rows at a time in LDS as
-//   Q[k][jx] = conj(cx[jx]) w(row),   C[k][jy] = conj(cy[jy]),
-// finds where each local cell starts in the staged chunk (the first row of
-// each occupied cell marks it, a block-wide suffix minimum fills the rest), and
-// every thread then sums, for the grid points it owns, the rows of the Ky
-// consecutive cells of each of the Kx cell columns that reach the point.
-// Each output point is written once by one thread, in a fixed order: no
-// atomics, deterministic, no per-nonzero index stream (the CSC is not built).
-// The uv density of a tile varies by orders of magnitude (empty corners, a
-// dense centre), so the work is cut into items of at most TILE_ITEM listed
-// rows: a dense tile is spread over several workgroups whose partial tiles
-// a second pass adds in a fixed order, and empty tiles are zeroed in groups.
+// listed under each tile its Ky x Kx window touches (1, 2 or 4 tiles), with
+// its anchor cell relative to the tile ("local cell": (TILE_Y + Ky - 1) x
+// (TILE_X + Kx - 1) cells, x-major, the list sorted by it).  One workgroup
+// owns one tile and stages its listed rows in LDS a chunk at a time.  Two
+// forms, chosen by the row width:
+//  - 8-slot rows (7 x 7 windows): a cell-indexed gather.  The chunk is
+//    staged as Q[k][jx] = conj(cx[jx]) w(row), C[k][jy] = conj(cy[jy]);
+//    where each local cell starts in the chunk is found (the first row of
+//    each occupied cell marks it, a block-wide suffix minimum fills the
+//    rest), and every thread sums, for the grid points it owns, the rows of
+//    the Ky consecutive cells of each of the Kx cell columns that reach it.
+//  - 16-slot rows (the w-projected 15 x 15 windows): on the matrix cores.
+//    Restricted to one tile the adjoint is a sum of rank-1 updates,
+//      out[y][x] = sum_rows conj(cy[y - dy]) * (w(row) conj(cx[x - dx])),
+//    (dy, dx) the row's window origin relative to the tile: a GEMM whose
+//    inner dimension runs over the tile's rows.  Each of the 4 waves owns
+//    one 16 x 16 quadrant, accumulated by v_mfma_f32_16x16x4f32 in FP32, two
+//    complex rows per instruction pair (K = 4 real terms, re/im split over
+//    the A/B lanes), over the rows whose window meets its quadrant.  A
+//    7 x 7 window fills under a fifth of such an update, which is why the
+//    narrow rows keep the gather.
+// Every output point is written once, by one thread, after a fixed-order
+// sum: no atomics, deterministic.  The uv density of a tile varies by
+// orders of magnitude (empty corners, a dense centre), so the work is cut
+// into items of at most TILE_ITEM listed rows: a dense tile is spread over
+// several workgroups whose partial tiles a second pass adds in a fixed
+// order, and empty tiles are zeroed in groups.
 constexpr int TILE_Y = 32, TILE_X = 32;
-constexpr int TILE_CH = ADJ_THREADS;                 // rows staged per chunk
 constexpr int TILE_PTS = TILE_Y * TILE_X / ADJ_THREADS;
-template <int KM>
-constexpr int tile_cells_max() { return (TILE_Y + KM - 1) * (TILE_X + KM - 1); }
-constexpr int TILE_ITEM = 4 * TILE_CH;               // listed rows per work item
+constexpr int TILE_ITEM = 1024;                      // listed rows per work item
 constexpr int TILE_EMPTY_GROUP = 8;                  // empty tiles zeroed per work item
 constexpr int FWD_ITEM = 1024;                       // rows per forward work item
 template <int KM>
 constexpr int fwd_win() { return (TILE_Y + KM - 1) * (TILE_X + KM - 1); }   // forward window (complex)
+constexpr int TILE_GCH = ADJ_THREADS;                                        // rows staged per chunk (gather)
+template <int KM>
+constexpr int tile_cells_max() { return (TILE_Y + KM - 1) * (TILE_X + KM - 1); }
 
-// LDS of a tile workgroup (dynamic shared memory): the staged rows with an
+// LDS of a tile workgroup (dynamic shared memory).  Matrix-core form
+// (16-slot rows): the staged C and B slots of a chunk, then each staged
+// row's window origin.  Gather form (8-slot rows): the staged rows with an
 // odd stride of Kx + Ky + 1 complex values (conj(cx) w, then conj(cy)), so
-// lanes reading consecutive rows hit different banks; their local cells; the
-// cell start offsets.
-__host__ __device__ inline size_t tile_lds_bytes(int Ky, int Kx) {
-  return (size_t)TILE_CH * (Ky + Kx + 1) * 8 + (size_t)(TILE_CH + ADJ_THREADS / 64) * 4 +
+// lanes reading consecutive rows hit different banks; their local cells;
+// the cell start offsets.
+__host__ __device__ inline size_t tile_lds_bytes(int Ky, int Kx);
+__host__ __device__ inline size_t tile_lds_bytes_gather(int Ky, int Kx) {
+  return (size_t)TILE_GCH * (Ky + Kx + 1) * 8 + (size_t)(TILE_GCH + ADJ_THREADS / 64) * 4 +
          ((size_t)(TILE_Y + Ky - 1) * (TILE_X + Kx - 1) + 1) * 4;
 }
+
+// LDS of an adjoint workgroup that may run the CSC unit path (bytes)
+constexpr size_t ADJ_CSC_LDS_BYTES = ADJ_LDS * sizeof(float2);
+
+inline bool tile_grid_ok(int Kty, int Ktx, int Ky, int Kx) {
+  return Kty % TILE_Y == 0 && Ktx % TILE_X == 0 && Kty >= TILE_Y + Ky - 1 && Ktx >= TILE_X + Kx - 1;
+}
+
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// Matrix-core tile layout in LDS (16-slot rows).  Staged row i's slots sit
+// at [i * TM_STRIDE + TM_GAP, + 16) of the C and B arrays, with TM_GAP zero
+// slots before the first row and after every row: a lane reading slot
+// jy = qy0 + lane - dy of a row whose window meets its quadrant lands in
+// [-15, 30] around the row's first slot, inside its values or the zero gaps,
+// so the reads need no bounds checks.
+constexpr int TM_GAP = 16, TM_STRIDE = 32;
+template <int CH>
+__host__ __device__ constexpr size_t tile_mfma_lds_bytes() {
+  return 2 * (size_t)(CH * TM_STRIDE + TM_GAP) * 8 + (size_t)CH * 4 + (size_t)4 * (CH + 4) * 4;
+}
+constexpr int TILE_MCH = 64;                          // rows staged per chunk (matrix cores)
+
+__host__ __device__ inline size_t tile_lds_bytes(int Ky, int Kx) {
+  return (Ky > 8 || Kx > 8) ? tile_mfma_lds_bytes<TILE_MCH>() : tile_lds_bytes_gather(Ky, Kx);
+}
+
+template <int KM, class W>
+__device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&& w, float2* __restrict__ out,
+                                                    unsigned char* lds) {
+  static_assert(ADJ_THREADS == 256, "one wave per 16 x 16 quadrant of a 32 x 32 tile");
+  static_assert(KM <= 16 && TM_GAP >= 15 && TM_STRIDE - 16 >= 15, "zero gaps cover a 16-lane read");
+  constexpr int CH = TILE_MCH;
+  constexpr int NSLOT = CH * TM_STRIDE + TM_GAP;
+  float2* C = reinterpret_cast<float2*>(lds);
+  float2* B = C + NSLOT;
+  int* meta = reinterpret_cast<int*>(B + NSLOT);       // [CH] window origin of staged row i
+  int* wl = meta + CH;                                 // [4][CH + 4] this wave's rows, as slot offsets
+  const int4 it = g.items[item];
+  if (it.w == -2) {   // a group of empty tiles: zeros
+    for (int e = it.y; e < it.z; ++e) {
+      const int t = g.empty_tiles[e];
+      const int ty = t % g.tiles_y, tx = t / g.tiles_y;
+#pragma unroll
+      for (int q = 0; q < TILE_PTS; ++q) {
+        const int p = threadIdx.x + q * ADJ_THREADS;
+        out[(size_t)(tx * TILE_X + p / TILE_Y) * g.Kty + ty * TILE_Y + p % TILE_Y] = make_float2(0.f, 0.f);
+      }
+    }
+    return;
+  }
+  const int t = it.x;
+  const int ty = t % g.tiles_y, tx = t / g.tiles_y;
+  const int ncy = TILE_Y + g.Ky - 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15;
+  const bool odd = (lane >> 4) & 1;
+  const int half = lane >> 5;                          // A/B lanes 0-31: first row of a pair, 32-63: second
+  const int qy0 = (wv & 1) * 16, qx0 = (wv >> 1) * 16;
+  int* mine = wl + wv * (CH + 4);
+  // the gaps (and unused slots) are zero for the whole item
+  for (int k = threadIdx.x; k < NSLOT; k += ADJ_THREADS) {
+    C[k] = make_float2(0.f, 0.f);
+    B[k] = make_float2(0.f, 0.f);
+  }
+  mfma_f4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+  for (int c0 = it.y; c0 < it.z; c0 += CH) {
+    const int n = min(CH, it.z - c0);
+    const int i = threadIdx.x;
+    __syncthreads();   // the previous chunk's reads are done (and the zeroing)
+    if (i < n) {
+      const int r = g.tile_rows[c0 + i];
+      const int cell = g.tile_cell[c0 + i];
+      const float2 v = w(r);
+      float2 cy[KM], cx[KM];
+      sep_load<KM>(g, r, cy, cx);
+      float2* qc = C + i * TM_STRIDE + TM_GAP;
+      float2* qb = B + i * TM_STRIDE + TM_GAP;
+#pragma unroll
+      for (int j = 0; j < KM; ++j) {
+        qc[j] = j < g.Ky ? make_float2(cy[j].x, -cy[j].y) : make_float2(0.f, 0.f);       // conj(cy)
+        qb[j] = j < g.Kx ? make_float2(cx[j].x * v.x + cx[j].y * v.y, cx[j].x * v.y - cx[j].y * v.x)
+                         : make_float2(0.f, 0.f);                                         // conj(cx) v
+      }
+      meta[i] = (cell % ncy - (g.Ky - 1) + 32) | ((cell / ncy - (g.Kx - 1) + 32) << 8);
+    }
+    __syncthreads();
+    // this wave's rows (window meets its quadrant), listed as the slot
+    // offsets of lane 0's reads: C at row + qy0 - dy, B at row + qx0 - dx
+    int nh = 0;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+      const int k = b0 + lane;
+      bool hit = false;
+      int e = 0;
+      if (k < n) {
+        const int m = meta[k];
+        const int dy = (m & 0xff) - 32, dx = ((m >> 8) & 0xff) - 32;
+        hit = dy < qy0 + 16 && dy + g.Ky > qy0 && dx < qx0 + 16 && dx + g.Kx > qx0;
+        const int base = k * TM_STRIDE + TM_GAP;
+        e = (base + qy0 - dy) | ((base + qx0 - dx) << 16);
+      }
+      const unsigned long long mask = __ballot(hit);
+      const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
+      if (hit) mine[nh + pos] = e;
+      nh += __popcll(mask);
+    }
+    // rows missing from the last group of four read the leading zero gap
+    if (lane < 3) mine[nh + lane] = 0;
+    __syncthreads();
+    // rank-1 updates, two rows per instruction pair and two pairs per step
+    // (their LDS reads issued together): A lane (i, k) holds re / im (k
+    // even / odd) of C at its row's offset + i, B lane (j, k) the matching
+    // re / -im or im / re of B, so cre += Re(C B^T), cim += Im(C B^T)
+    for (int p = 0; p < nh; p += 4) {
+      const int e0 = mine[p + half], e1 = mine[p + 2 + half];
+      const float2 c0 = C[(e0 & 0xffff) + li], b0 = B[(e0 >> 16) + li];
+      const float2 c1 = C[(e1 & 0xffff) + li], b1 = B[(e1 >> 16) + li];
+      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c0.y : c0.x, odd ? -b0.y : b0.x, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c0.y : c0.x, odd ? b0.x : b0.y, cim, 0, 0, 0);
+      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c1.y : c1.x, odd ? -b1.y : b1.x, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c1.y : c1.x, odd ? b1.x : b1.y, cim, 0, 0, 0);
+    }
+  }
+  // lane (j, k) holds rows 4k .. 4k + 3 of column j of the quadrant
+  const int lx = qx0 + li, ly0 = qy0 + 4 * (lane >> 4);
+  if (it.w >= 0) {   // one of several items of this tile: partial sums
+    float2* part = g.partial + (size_t)it.w * (TILE_Y * TILE_X) + lx * TILE_Y + ly0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part[r] = make_float2(cre[r], cim[r]);
+    return;
+  }
+  float2* o = out + (size_t)(tx * TILE_X + lx) * g.Kty + ty * TILE_Y + ly0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) o[r] = make_float2(cre[r], cim[r]);
+}
+
+// ---- the cell-indexed gather (8-slot rows) ----
 
 struct TileView {
   float2* qc;
@@ -288,16 +440,9 @@ __device__ __forceinline__ TileView tile_view(const GDev& g, unsigned char* lds)
   TileView v;
   v.stride = g.Ky + g.Kx + 1;
   v.qc = reinterpret_cast<float2*>(lds);
-  v.cell = reinterpret_cast<int*>(lds + (size_t)TILE_CH * v.stride * 8);
-  v.start = v.cell + TILE_CH + ADJ_THREADS / 64;   // cell[TILE_CH], scan scratch, start
+  v.cell = reinterpret_cast<int*>(lds + (size_t)TILE_GCH * v.stride * 8);
+  v.start = v.cell + TILE_GCH + ADJ_THREADS / 64;   // cell[TILE_GCH], scan scratch, start
   return v;
-}
-
-// LDS of an adjoint workgroup that may run the CSC unit path (bytes)
-constexpr size_t ADJ_CSC_LDS_BYTES = ADJ_LDS * sizeof(float2);
-
-inline bool tile_grid_ok(int Kty, int Ktx, int Ky, int Kx) {
-  return Kty % TILE_Y == 0 && Ktx % TILE_X == 0 && Kty >= TILE_Y + Ky - 1 && Ktx >= TILE_X + Kx - 1;
 }
 
 // a[c] = min(a[c], a[c+1], ..., a[n-1]) for c < n, by the ADJ_THREADS
@@ -336,8 +481,8 @@ __device__ __forceinline__ void tile_suffix_min(int* a, int n, int* wsum) {
 }
 
 template <int KM, class W>
-__device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, float2* __restrict__ out,
-                                               unsigned char* lds) {
+__device__ __forceinline__ void g_adjoint_tile_gather(const GDev& g, int item, W&& w, float2* __restrict__ out,
+                                                      unsigned char* lds) {
   const TileView s = tile_view(g, lds);
   const int4 it = g.items[item];
   if (it.w == -2) {   // a group of empty tiles: zeros
@@ -367,12 +512,12 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
     r_next = g.tile_rows[p0 + i];
     cell_next = g.tile_cell[p0 + i];
   }
-  for (int c0 = p0; c0 < p1; c0 += TILE_CH) {
-    const int n = min(TILE_CH, p1 - c0);
+  for (int c0 = p0; c0 < p1; c0 += TILE_GCH) {
+    const int n = min(TILE_GCH, p1 - c0);
     const int r = r_next, rcell = cell_next;
-    if (c0 + TILE_CH + i < p1) {
-      r_next = g.tile_rows[c0 + TILE_CH + i];
-      cell_next = g.tile_cell[c0 + TILE_CH + i];
+    if (c0 + TILE_GCH + i < p1) {
+      r_next = g.tile_rows[c0 + TILE_GCH + i];
+      cell_next = g.tile_cell[c0 + TILE_GCH + i];
     }
     if (i < n) {
       const float2 v = w(r);
@@ -395,7 +540,7 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
     // occupied cell marks it, then a block-wide suffix minimum fills the rest
     if (i < n && (i == 0 || s.cell[i - 1] != s.cell[i])) s.start[s.cell[i]] = i;
     __syncthreads();
-    tile_suffix_min<KM>(s.start, ncells + 1, s.cell + TILE_CH);
+    tile_suffix_min<KM>(s.start, ncells + 1, s.cell + TILE_GCH);
     // occupied cell columns of this chunk (cells are sorted, x-major)
     const int cxmin = s.cell[0] / ncy, cxmax = s.cell[n - 1] / ncy;
 #pragma unroll
@@ -431,6 +576,16 @@ __device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, f
     const int ly = p % TILE_Y, lx = p / TILE_Y;
     out[(size_t)(tx * TILE_X + lx) * g.Kty + ty * TILE_Y + ly] = acc[q];
   }
+}
+
+
+template <int KM, class W>
+__device__ __forceinline__ void g_adjoint_tile(const GDev& g, int item, W&& w, float2* __restrict__ out,
+                                               unsigned char* lds) {
+  if constexpr (KM > 8)
+    g_adjoint_tile_mfma<KM>(g, item, w, out, lds);
+  else
+    g_adjoint_tile_gather<KM>(g, item, w, out, lds);
 }
 
 // sum of the partial tiles of split tile r, in slot order

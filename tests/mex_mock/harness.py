@@ -80,9 +80,13 @@ class Mex:
             A.sort_indices()
             cplx = np.iscomplexobj(A.data)
             s = m.mxCreateSparse(A.shape[0], A.shape[1], max(A.nnz, 1), int(cplx))
-            C.memmove(m.mxGetJc(s), np.ascontiguousarray(A.indptr, dtype=np.uint64).ctypes.data, 8 * (A.shape[1] + 1))
+            # the converted arrays are bound to names: a temporary's buffer
+            # would be freed before memmove reads it
+            jc = np.ascontiguousarray(A.indptr, dtype=np.uint64)
+            C.memmove(m.mxGetJc(s), jc.ctypes.data, jc.nbytes)
             if A.nnz:
-                C.memmove(m.mxGetIr(s), np.ascontiguousarray(A.indices, dtype=np.uint64).ctypes.data, 8 * A.nnz)
+                ir = np.ascontiguousarray(A.indices, dtype=np.uint64)
+                C.memmove(m.mxGetIr(s), ir.ctypes.data, ir.nbytes)
                 d = np.ascontiguousarray(A.data, dtype=np.complex128 if cplx else np.float64)
                 C.memmove(m.mxGetData(s), d.ctypes.data, d.nbytes)
             return s
@@ -91,9 +95,9 @@ class Mex:
         dims = a.shape if a.ndim >= 2 else ((a.size, 1) if a.ndim == 1 else (1, 1))
         d = (C.c_size_t * len(dims))(*dims)
         mxa = m.mxCreateNumericArray(len(dims), d, _CLS_DOUBLE, int(cplx))
-        buf = np.asarray(a, dtype=np.complex128 if cplx else np.float64).ravel(order='F')
+        buf = np.ascontiguousarray(np.asarray(a, dtype=np.complex128 if cplx else np.float64).ravel(order='F'))
         if buf.size:
-            C.memmove(m.mxGetData(mxa), np.ascontiguousarray(buf).ctypes.data, buf.nbytes)
+            C.memmove(m.mxGetData(mxa), buf.ctypes.data, buf.nbytes)
         return mxa
 
     def from_mx(self, p):

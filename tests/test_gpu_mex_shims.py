@@ -15,7 +15,7 @@ import harness  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-NY, NX, L = 32, 40, 2
+NY, NX, L = 32, 64, 2
 BLOCKS = (300, 260)
 
 
@@ -151,7 +151,7 @@ def test_fhs_update_dual_lowrankness(mex):
     base = rng.random((M * N, 2)) @ rng.random((2, c))
     f32 = lambda a: a.astype(np.float32).astype(np.float64)      # noqa: E731  (the device holds FP32)
     xhat = f32((base + 0.05 * rng.random((M * N, c))).reshape((M, N, c), order='F'))
-    v0 = f32(0.1 * rng.standard_normal((M * N, c)))
+    v0 = f32(0.01 * rng.standard_normal((M * N, c)))
     w = f32(np.outer(np.linspace(0.2, 1.0, M), np.linspace(1.0, 0.3, N)))
     weights0 = np.linspace(1.0, 2.0, c)
     beta0 = 0.8

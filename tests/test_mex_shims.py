@@ -64,3 +64,23 @@ def test_update_epsilon_shim_matches_the_oracle(mex):
     assert e_ref != eps and t_ref[1][1] == 95      # the cases changed something, the waiting block nothing
     with pytest.raises(harness.MexError, match='fhs:args'):
         mex.call('update_epsilon', 2, eps, 100, tb)
+
+
+def test_harness_round_trips_arrays_cells_and_sparse(mex):
+    """The stand-in's conversions keep MATLAB's column-major layout."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(0)
+    for a in (rng.random((3, 4, 2)), rng.random((5, 1)) + 1j * rng.random((5, 1)), np.zeros((0, 0))):
+        b = mex.from_mx(mex.to_mx(a))
+        assert np.array_equal(np.asarray(b).reshape(a.shape), a)
+    cell = mex.from_mx(mex.to_mx([[1.0, rng.random((2, 2))], [np.ones((3, 1))]]))
+    assert cell[0][0] == 1.0 and cell[0][1].shape == (2, 2) and cell[1][0].shape == (3, 1)
+    A = (sp.random(40, 30, density=0.1, format='csr', random_state=2) * (1 - 2j)).tocsc()
+    A.sort_indices()
+    p = mex.to_mx(A)
+    jc = np.ctypeslib.as_array(C.cast(mex.m.mxGetJc(p), C.POINTER(C.c_uint64)), shape=(31,))
+    ir = np.ctypeslib.as_array(C.cast(mex.m.mxGetIr(p), C.POINTER(C.c_uint64)), shape=(A.nnz,))
+    z = np.ctypeslib.as_array(C.cast(mex.m.mxGetData(p), C.POINTER(C.c_double)), shape=(2 * A.nnz,))
+    assert np.array_equal(jc, A.indptr) and np.array_equal(ir, A.indices)
+    assert np.array_equal(z[0::2] + 1j * z[1::2], A.data)
+    mex.m.mxDestroyArray(p)
