@@ -641,6 +641,60 @@ __global__ __launch_bounds__(256) void k_l21(SaraGeom G, float* __restrict__ v1,
   }
 }
 
+// The same dual step (mode 0) four consecutive coefficients per thread with
+// 16-byte loads and stores: every channel's vector load of a step is issued
+// before the norms need them.  For s, the Dirac offset and Ny*Nx multiples of
+// 4 (a group of four is then wholly Dirac or wholly wavelet).
+__global__ __launch_bounds__(256) void k_l21_v4(SaraGeom G, float* __restrict__ v1,
+                                                const float* __restrict__ xhat, int L,
+                                                const float* __restrict__ w, double beta) {
+  constexpr int LREG = 16;
+  const int64_t s4 = G.s / 4;
+  const int64_t dstart = G.boff[G.nb];
+  const int64_t npix = (int64_t)G.Ny * G.Nx;
+  float4* __restrict__ v = reinterpret_cast<float4*>(v1);
+  for (int64_t c4 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c4 < s4;
+       c4 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = 4 * c4;
+    const bool dir = G.dirac && c >= dstart && xhat;
+    float4 rv[LREG];
+    double ss[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int l = 0; l < LREG; ++l) {
+      if (l < L) {
+        float4 r = v[(int64_t)l * s4 + c4];
+        if (dir) {
+          const float4 xv = *reinterpret_cast<const float4*>(xhat + (int64_t)l * npix + (c - dstart));
+          r.x += xv.x * G.norm;
+          r.y += xv.y * G.norm;
+          r.z += xv.z * G.norm;
+          r.w += xv.w * G.norm;
+        }
+        rv[l] = r;
+        ss[0] += (double)r.x * r.x;
+        ss[1] += (double)r.y * r.y;
+        ss[2] += (double)r.z * r.z;
+        ss[3] += (double)r.w * r.w;
+      }
+    }
+    float4 wv = make_float4(1.f, 1.f, 1.f, 1.f);
+    if (w) wv = *reinterpret_cast<const float4*>(w + c);
+    const float wa[4] = {wv.x, wv.y, wv.z, wv.w};
+    float f[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double l2 = sqrt(ss[j]);
+      f[j] = (float)(fmax(l2 - beta * (double)wa[j], 0.0) / (l2 + 2.220446049250313e-16));
+    }
+#pragma unroll
+    for (int l = 0; l < LREG; ++l)
+      if (l < L) {
+        const float4 r = rv[l];
+        v[(int64_t)l * s4 + c4] = make_float4(r.x - f[0] * r.x, r.y - f[1] * r.y, r.z - f[2] * r.z, r.w - f[3] * r.w);
+      }
+  }
+}
+
 __global__ void k_f64_to_f32(const double* __restrict__ a, int64_t n, float* __restrict__ b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
@@ -724,8 +778,13 @@ void sara_dual_step(fhs_ctx* ctx, const fhs_sara* S, float* v1, const float* xha
   if (G.nb) sara_analysis_wavelets(ctx, S, xhat, L, v1);
   {
     Scope sc(ctx, "l21_threshold");
-    k_l21<<<cdiv(G.s, 256) < 2048 ? cdiv(G.s, 256) : 2048, 256, 0, ctx->stream>>>(G, v1, xhat, L, w, beta, 0,
-                                                                                   nullptr);
+    const bool v4 = L <= 16 && G.s % 4 == 0 && G.boff[G.nb] % 4 == 0 && ((int64_t)G.Ny * G.Nx) % 4 == 0 &&
+                    (((uintptr_t)v1 | (uintptr_t)xhat | (uintptr_t)w) % 16) == 0;
+    if (v4)
+      k_l21_v4<<<std::min<int64_t>(cdiv(G.s / 4, 256), 2048), 256, 0, ctx->stream>>>(G, v1, xhat, L, w, beta);
+    else
+      k_l21<<<cdiv(G.s, 256) < 2048 ? cdiv(G.s, 256) : 2048, 256, 0, ctx->stream>>>(G, v1, xhat, L, w, beta, 0,
+                                                                                     nullptr);
     check_launch();
   }
   sara_synthesis(ctx, S, v1, L, sigma1, g1);
