@@ -69,6 +69,7 @@ PROTOTYPES = {
     'fhs_data_init_fx_old': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_data_fx_old_to_state': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_data_info': [c_void_p, c_int_p, c_int_p, c_int_p, c_i64_p, c_i64_p, c_int_p],
+    'fhs_precond_uniform_weights': [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p],
     'fhs_data_plan': [c_void_p, C.POINTER(c_void_p)],
     'fhs_nufft_info': [c_void_p, c_int_p, c_int_p, c_int_p, c_int_p],
     'fhs_update_dual_data_fidelity': [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -233,6 +233,18 @@ def wkernel_table(N, K, fov, w_max, S, keep=1.0, dw=0.05):
     return out
 
 
+def uniform_weights_gpu(ctx, u, v, Kty, Ktx):
+    """``uniform_weights`` on the GPU (``fhs_precond_uniform_weights``):
+    float32 device tensor of 1/(points in the cell)."""
+    dev = ctx.device
+    ud = torch.from_numpy(np.ascontiguousarray(u, dtype=np.float64)).to(dev)
+    vd = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
+    out = torch.empty(len(ud), dtype=torch.float32, device=dev)
+    ctx.check(lib().fhs_precond_uniform_weights(ctx.h, _ptr(ud), _ptr(vd), len(ud), int(Kty), int(Ktx), _ptr(out)),
+              'precond_uniform_weights')
+    return out
+
+
 def uniform_weights(u, v, Kty, Ktx):
     """Preconditioning weights aW = 1/(number of uv points in the same cell of
     the oversampled grid): util_gen_preconditioning_matrix with

@@ -19,7 +19,7 @@ from ._lib import lib
 from .core import GMatrix, _ptr
 from .data_fidelity import DataOperator
 from .lowrank import LowRankPrior
-from .nufft import Nufft2D, uniform_weights
+from .nufft import Nufft2D, uniform_weights, uniform_weights_gpu
 from .sara import SaraDictionary
 from .synth import add_noise
 
@@ -100,8 +100,11 @@ class Measurement:
         self.D = DataOperator(ctx, self.plan, self.G, P.blocks)
         self.y = torch.from_numpy(np.concatenate(self.y_host).astype(np.complex64)).cuda()
         if precond:
+            # device weights from the GPU counter; the host copy (FP64, same
+            # cells) serves host-side consumers such as residual checks
+            self.pU = torch.cat([uniform_weights_gpu(ctx, self.u[l], self.v[l], self.nufft.Kty, self.nufft.Ktx)
+                                 for l in range(P.L)])
             self.pU_host = [uniform_weights(self.u[l], self.v[l], self.nufft.Kty, self.nufft.Ktx) for l in range(P.L)]
-            self.pU = torch.from_numpy(np.concatenate(self.pU_host).astype(np.float32)).cuda()
         else:
             self.pU_host = [np.ones(len(u)) for u in self.u]
             self.pU = None

@@ -172,6 +172,16 @@ int fhs_apply_scaled_fourier_adjoint(fhs_ctx* ctx, const fhs_nufft* p,
                                      const void* g, int L, const float* scale,
                                      float* out);
 
+/* ------------------------------------------------------ preconditioner -- */
+/* aW = util_gen_preconditioning_matrix(u, v, param_precond) with uniform
+ * weighting (gen_uniform_weight_matrix, imaging/default_parameters.json:47-51;
+ * called at lib/operators/util_gen_measurement_operator_synth.m:74 and
+ * lib/operators/util_load_data_gen_measurement_operator.m:116): 1 / (number
+ * of points in the same cell of the Kty x Ktx grid).  u, v: device double[M]
+ * in radians ([-pi, pi]); aW: device float[M] out.  Deterministic. */
+int fhs_precond_uniform_weights(fhs_ctx* ctx, const double* u, const double* v, int64_t M, int Kty, int Ktx,
+                                float* aW);
+
 /* ---------------------------------------------------- l2 ball / duals -- */
 /* p = y + (x - y) * min(eps / ||x - y||, 1)      src/proj_l2ball.m:20-22
  * x, y, p: complex[n] (p may alias x). */

@@ -153,3 +153,23 @@ def test_interp_coefficients_gpu(ctx, Ny, Nx):
                 row[(ky0 + jy) % Kty + Kty * ((kx0 + jx) % nf.Ktx)] += cy0[m, jy] * cx0[m, jx]
         ref = orc.G.getrow(m).toarray().ravel()
         assert np.abs(row - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('N,M', [(64, 5000), (512, 200000)])
+def test_uniform_weights_gpu_match_oracle(ctx, N, M):
+    """fhs_precond_uniform_weights against the oracle's count (exact: 1/count
+    of integers) and the product's host rule, with points stacked in a few
+    cells and on the grid's edge so that counts above one occur."""
+    from fhs.nufft import uniform_weights, uniform_weights_gpu
+    from oracle.nufft import uniform_precond_weights
+    rng = np.random.default_rng(N)
+    u = rng.normal(0, np.pi / 4, M).clip(-np.pi, np.pi)
+    v = rng.normal(0, np.pi / 4, M).clip(-np.pi, np.pi)
+    u[:50], v[:50] = 0.1, -0.2              # 50 points in one cell
+    u[50:60], v[50:60] = np.pi, -np.pi      # the edge wraps to cell 0
+    K = 2 * N
+    got = uniform_weights_gpu(ctx, u, v, K, K).cpu().numpy()
+    ref = uniform_precond_weights(u, v, K, K)
+    assert np.array_equal(got, ref.astype(np.float32))
+    assert np.array_equal(got, uniform_weights(u, v, K, K).astype(np.float32))
+    assert round(1 / float(got[0])) >= 50 and np.all(got[:50] == got[0])    # the stacked cell
