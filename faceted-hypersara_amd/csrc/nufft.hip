@@ -4,17 +4,21 @@
 // The grid is Kty x Ktx (column-major, y contiguous); the image Ny x Nx sits
 // in its top-left corner.  Each transform is two passes through HBM:
 //   forward : (1) y-pass: one workgroup per image column, scale by sn, zero
-//                 pad to Kty and FFT in LDS -> T (Kty x Nx, only Nx columns
-//                 exist, the rest are zero);
+//                 pad to Kty and FFT -> T (Kty x Nx, only Nx columns exist,
+//                 the rest are zero);
 //             (2) x-pass: one workgroup per B consecutive grid rows, FFT of
 //                 length Ktx over the Nx non-zero columns of T -> Fx.
 //   adjoint : (1') x-pass inverse keeping the first Nx outputs -> T;
 //             (2') y-pass inverse, keep Ny outputs, * sn, real part, * scale.
-// The x-pass reads and writes B consecutive complex values per grid column
-// (B = 16 -> 128-byte segments, B = 8 -> 64, B = 4 -> 32) and keeps B transforms interleaved in LDS
-// (s[i*B + b]) so that consecutive lanes touch consecutive LDS words.
-// FFTs are radix-4 Stockham (autosort) with one radix-2 stage when log2 n is
-// odd; twiddles come from an FP64-computed table exp(-2 pi i k/n).
+// The y-passes use a register-resident radix-16 Stockham FFT whose first
+// stage reads the column from global memory and whose last stage writes it
+// (two LDS round trips at 4096 points).  The x-passes read and write B
+// consecutive complex values per grid column (B = 16 -> 128-byte segments,
+// B = 8 -> 64, B = 4 -> 32), keep the B transforms interleaved in LDS
+// (s[i*B + b]) so that consecutive lanes touch consecutive LDS words, and
+// run radix-4 Stockham stages in LDS (one radix-2 stage when log2 n is
+// odd): measured faster than the register form for these strided passes.
+// Twiddles come from an FP64-computed table exp(-2 pi i k/n).
 #include <cmath>
 
 #include "common.hpp"
@@ -26,6 +30,7 @@ namespace {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
+// ---- LDS radix-4 Stockham FFT (the x-passes) --------------------------------
 template <int N, int B, int NT, bool INV>
 __device__ __forceinline__ void fft_lds(float2* s, const float2* tw) {
   const int tid = threadIdx.x;
@@ -110,8 +115,6 @@ __device__ __forceinline__ void fft_lds(float2* s, const float2* tw) {
 // N = 4096 the table would halve their occupancy.
 template <int N>
 constexpr int TW_LDS = (N < 4 || N > 2048) ? 1 : 3 * N / 4;
-template <int N>
-constexpr int TW_LDS_Y = N < 4 ? 1 : 3 * N / 4;
 template <int N, int NT, int TWN>
 __device__ __forceinline__ const float2* stage_twiddles(const float2* __restrict__ tw, float2* tws) {
   if constexpr (TWN == 1) return tw;
@@ -119,8 +122,144 @@ __device__ __forceinline__ const float2* stage_twiddles(const float2* __restrict
   return tws;
 }
 
+// ---- register-resident Stockham FFT ----------------------------------------
+// A length-N transform (N a power of two) is done in stages of radix R = 16
+// (then one smaller radix for what remains: 2048 = 16 16 8).  A stage of
+// sub-transform size NS takes butterfly j (0 <= j < N/R) with k = j mod NS:
+//   v[r] = in[j + r N/R] * W_N^(r k N/(NS R)),  r < R,   (no twiddle at NS = 1)
+//   v <- DFT_R(v),  out[(j / NS) NS R + k + r NS] = v[r],
+// with W_N = exp(-2 pi i / N) (conjugated for the inverse).  Each thread keeps
+// its butterfly's R values in registers; the first stage reads its inputs
+// from global memory and the last writes its outputs there (first stage:
+// n = j + r N/R; last stage: n = j + r N/R as well, since NS R = N), so a
+// 4096-point transform makes two LDS round trips (three barriers) instead
+// of the six of radix 4.  B transforms of a workgroup are interleaved in LDS
+// (slot n B + b), with one pad slot per 16 rows so that the stride-R writes
+// of the early stages do not all fall in one bank.
+// cos / sin of 2 pi k / 16, k = 0..7
+__device__ constexpr float kC16[8] = {1.f, 0.923879532511287f, 0.707106781186548f, 0.382683432365090f,
+                                      0.f, -0.382683432365090f, -0.707106781186548f, -0.923879532511287f};
+__device__ constexpr float kS16[8] = {0.f, 0.382683432365090f, 0.707106781186548f, 0.923879532511287f,
+                                      1.f, 0.923879532511287f, 0.707106781186548f, 0.382683432365090f};
+
+// in-place DFT of R = 1, 2, 4, 8 or 16 registers (radix-2 decimation in time)
+template <int R, bool INV>
+__device__ __forceinline__ void dft_reg(float2* v) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R > 2) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    dft_reg<R / 2, INV>(e);
+    dft_reg<R / 2, INV>(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const int q = k * (16 / R);   // W_R^k = W_16^q
+      float2 t;
+      if (q == 0) {
+        t = o[k];
+      } else if (q == 4) {          // -i forward, +i inverse
+        t = INV ? make_float2(-o[k].y, o[k].x) : make_float2(o[k].y, -o[k].x);
+      } else {
+        const float2 w = make_float2(kC16[q], INV ? kS16[q] : -kS16[q]);
+        t = cmul(o[k], w);
+      }
+      v[k] = cadd(e[k], t);
+      v[k + R / 2] = csub(e[k], t);
+    }
+  }
+}
+
+__host__ __device__ constexpr int fft_pad(int n) { return n + (n >> 4); }
+
+template <int N, int B>
+constexpr int fft_lds_slots() { return fft_pad(N) * B + B; }
+
 template <int N>
-constexpr int nt_contig() { return N / 4 < 64 ? 64 : (N / 4 > 1024 ? 1024 : N / 4); }
+constexpr int fft_radix(int ns) { return N / ns >= 16 ? 16 : N / ns; }
+
+// one stage; load(q, r, b, n) / store(q, r, b, n, v) serve the first / last
+// stage (q: the thread's butterfly slot, r: its value index)
+template <int N, int R, int NS, int B, int NT, bool INV, bool FIRST, bool LAST, class Load, class Store>
+__device__ __forceinline__ void fft_stage(float2* s, const float2* __restrict__ tw, int tws, Load& load,
+                                          Store& store) {
+  constexpr int NBF = B * N / R;
+  constexpr int PER = (NBF + NT - 1) / NT;
+  constexpr int TS = N / (NS * R);
+  float2 v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int u = threadIdx.x + q * NT;
+    if (u < NBF) {
+      const int b = u % B, j = u / B;
+      const int k = j & (NS - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int n = j + r * (N / R);
+        v[q][r] = FIRST ? load(q, r, b, n) : s[fft_pad(n) * B + b];
+      }
+      if (NS > 1 && k) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[(r * k * TS) * tws];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft_reg<R, INV>(v[q]);
+    }
+  }
+  if constexpr (!FIRST && !LAST) __syncthreads();   // in place: every read of the stage before its writes
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int u = threadIdx.x + q * NT;
+    if (u < NBF) {
+      const int b = u % B, j = u / B;
+      const int k = j & (NS - 1);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int n = base + r * NS;
+        if constexpr (LAST)
+          store(q, r, b, n, v[q][r]);
+        else
+          s[fft_pad(n) * B + b] = v[q][r];
+      }
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+}
+
+template <int N, int NS, int B, int NT, bool INV, bool FIRST, class Load, class Store>
+__device__ __forceinline__ void fft_stages(float2* s, const float2* __restrict__ tw, int tws, Load& load,
+                                           Store& store) {
+  constexpr int R = fft_radix<N>(NS);
+  constexpr bool LAST = NS * R == N;
+  fft_stage<N, R, NS, B, NT, INV, FIRST, LAST>(s, tw, tws, load, store);
+  if constexpr (!LAST) fft_stages<N, NS * R, B, NT, INV, false>(s, tw, tws, load, store);
+}
+
+// B transforms of length N: inputs from load, outputs to store.  tw is the
+// twiddle table exp(-2 pi i m / (N tws)) read at stride tws (tws = 2 for the
+// half-length transforms of the split x-passes).  s: fft_lds_slots<N, B>()
+// complex values of LDS.  Must be called by all NT threads (it has barriers).
+template <int N, int B, int NT, bool INV, class Load, class Store>
+__device__ __forceinline__ void fft_reg(float2* s, const float2* __restrict__ tw, int tws, Load& load,
+                                        Store& store) {
+  fft_stages<N, 1, B, NT, INV, true>(s, tw, tws, load, store);
+}
+
+// threads of a workgroup running B transforms of length n: one radix-16
+// butterfly each (at least one wave, at most 1024)
+template <int n, int B>
+constexpr int fft_threads() { return B * n / 16 < 64 ? 64 : (B * n / 16 > 1024 ? 1024 : B * n / 16); }
+
 // grid rows per x-pass workgroup: 16 up to N = 512, 8 at N = 1024 (two
 // workgroups per CU), 4 from N = 2048 (64 KB of rows plus the twiddles, two
 // workgroups per CU; 128 KB and one per CU at N = 4096, where 2 rows, 16-byte
@@ -152,36 +291,29 @@ __device__ __forceinline__ int xpass_row0() {
   }
 }
 
-// (1) forward y-pass
+// threads of a y-pass workgroup of an N-point grid
 template <int N>
-__global__ __launch_bounds__(nt_contig<N>()) void k_fwd_y(const float* __restrict__ x,
-                                                         const float* __restrict__ sy,
-                                                         const float* __restrict__ sx, int Ny, int Nx,
-                                                         const float2* __restrict__ tw,
-                                                         float2* __restrict__ T) {
-  constexpr int NT = nt_contig<N>();
-  __shared__ float2 s[N];
-  __shared__ float2 tws[TW_LDS_Y<N>];
+constexpr int nt_y() { return fft_threads<N, 1>(); }
+
+// (1) forward y-pass: one workgroup per image column, scaled by sn, zero
+// padded to N = Kty
+template <int N>
+__global__ __launch_bounds__(nt_y<N>()) void k_fwd_y(const float* __restrict__ x,
+                                                              const float* __restrict__ sy,
+                                                              const float* __restrict__ sx, int Ny, int Nx,
+                                                              const float2* __restrict__ tw,
+                                                              float2* __restrict__ T) {
+  constexpr int NT = fft_threads<N, 1>();
+  __shared__ float2 s[fft_lds_slots<N, 1>()];
   const int ix = blockIdx.x, l = blockIdx.y;
-  const float2* twp = stage_twiddles<N, NT, TW_LDS_Y<N>>(tw, tws);
-  const float* xc = x + ((size_t)l * Nx + ix) * Ny;
+  const float* __restrict__ xc = x + ((size_t)l * Nx + ix) * Ny;
   const float scx = sx[ix];
-  constexpr int PER = (N + NT - 1) / NT;
-  float v[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = threadIdx.x + j * NT;
-    v[j] = i < Ny ? xc[i] * sy[i] * scx : 0.f;
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = threadIdx.x + j * NT;
-    if (i < N) s[i] = make_float2(v[j], 0.f);
-  }
-  __syncthreads();
-  fft_lds<N, 1, NT, false>(s, twp);
-  float2* out = T + ((size_t)l * Nx + ix) * N;
-  for (int i = threadIdx.x; i < N; i += NT) out[i] = s[i];
+  float2* __restrict__ out = T + ((size_t)l * Nx + ix) * N;
+  auto load = [&](int, int, int, int n) {
+    return n < Ny ? make_float2(xc[n] * sy[n] * scx, 0.f) : make_float2(0.f, 0.f);
+  };
+  auto store = [&](int, int, int, int n, float2 v) { out[n] = v; };
+  fft_reg<N, 1, NT, false>(s, tw, 1, load, store);
 }
 
 // (2) forward x-pass: N = Ktx
@@ -354,35 +486,23 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x_split(const float2* _
 
 // (2') adjoint y-pass: N = Kty, keep Ny outputs, real part, scale
 template <int N>
-__global__ __launch_bounds__(nt_contig<N>()) void k_adj_y(const float2* __restrict__ T,
-                                                         const float* __restrict__ sy,
-                                                         const float* __restrict__ sx, int Ny, int Nx,
-                                                         const float2* __restrict__ tw,
-                                                         const float* __restrict__ scale,
-                                                         float* __restrict__ out) {
-  constexpr int NT = nt_contig<N>();
-  __shared__ float2 s[N];
-  __shared__ float2 tws[TW_LDS_Y<N>];
+__global__ __launch_bounds__(nt_y<N>()) void k_adj_y(const float2* __restrict__ T,
+                                                              const float* __restrict__ sy,
+                                                              const float* __restrict__ sx, int Ny, int Nx,
+                                                              const float2* __restrict__ tw,
+                                                              const float* __restrict__ scale,
+                                                              float* __restrict__ out) {
+  constexpr int NT = fft_threads<N, 1>();
+  __shared__ float2 s[fft_lds_slots<N, 1>()];
   const int ix = blockIdx.x, l = blockIdx.y;
-  const float2* twp = stage_twiddles<N, NT, TW_LDS_Y<N>>(tw, tws);
-  const float2* Tc = T + ((size_t)l * Nx + ix) * N;
-  constexpr int PER = (N + NT - 1) / NT;
-  float2 v[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = threadIdx.x + j * NT;
-    v[j] = i < N ? Tc[i] : make_float2(0.f, 0.f);
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = threadIdx.x + j * NT;
-    if (i < N) s[i] = v[j];
-  }
-  __syncthreads();
-  fft_lds<N, 1, NT, true>(s, twp);
+  const float2* __restrict__ Tc = T + ((size_t)l * Nx + ix) * N;
   const float sc = (scale ? scale[l] : 1.f) * sx[ix];
-  float* o = out + ((size_t)l * Nx + ix) * Ny;
-  for (int i = threadIdx.x; i < Ny; i += NT) o[i] = sc * sy[i] * s[i].x;
+  float* __restrict__ o = out + ((size_t)l * Nx + ix) * Ny;
+  auto load = [&](int, int, int, int n) { return Tc[n]; };
+  auto store = [&](int, int, int, int n, float2 v) {
+    if (n < Ny) o[n] = sc * sy[n] * v.x;
+  };
+  fft_reg<N, 1, NT, true>(s, tw, 1, load, store);
 }
 
 template <template <int> class K, class... Args>
@@ -403,7 +523,7 @@ void dispatch_pow2(int n, Args&&... args) {
 template <int N>
 struct RunFwdY {
   static void run(hipStream_t st, int Nx, int L, const float* x, const fhs_nufft* p, float2* T) {
-    k_fwd_y<N><<<dim3(Nx, L), nt_contig<N>(), 0, st>>>(x, p->sy, p->sx, p->Ny, p->Nx, p->twy, T);
+    k_fwd_y<N><<<dim3(Nx, L), nt_y<N>(), 0, st>>>(x, p->sy, p->sx, p->Ny, p->Nx, p->twy, T);
   }
 };
 template <int N>
@@ -434,7 +554,7 @@ template <int N>
 struct RunAdjY {
   static void run(hipStream_t st, int Nx, int L, const float2* T, const fhs_nufft* p,
                   const float* scale, float* out) {
-    k_adj_y<N><<<dim3(Nx, L), nt_contig<N>(), 0, st>>>(T, p->sy, p->sx, p->Ny, p->Nx, p->twy, scale, out);
+    k_adj_y<N><<<dim3(Nx, L), nt_y<N>(), 0, st>>>(T, p->sy, p->sx, p->Ny, p->Nx, p->twy, scale, out);
   }
 };
 
