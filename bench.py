@@ -63,6 +63,8 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='keep the default launch shapes (the tuned ones give bit-identical results)')
     ap.add_argument('--w-max', type=float, default=None,
                     help='w-term: base w uniform in [0, W] wavelengths (fhs2048: 200), field of view --fov')
     ap.add_argument('--fov', type=float, default=0.08, help='field of view of the w-term (direction cosines)')
@@ -298,6 +300,7 @@ def cube_main(args):
         cfg['w_max'] = args.w_max
     cfg['fov'] = args.fov
     ctx = fhs.default_context(0)
+    ctx.set_autotune(not args.no_autotune)
     C = cfg['subcubes']
     t_setup = time.perf_counter()
     subs = []
@@ -421,6 +424,7 @@ def cube_main(args):
                      'avg_launch_ms': round(avg_ms, 5), 'algorithmic_bytes_per_launch': int(alg[dominant]),
                      'measured_on': 'sub-cube 0 (16 channels)'},
         'kernels': kernels,
+        'tuned_launch_shapes': ctx.autotune_report(),
         'solve': solve,
         'cpu_baseline': cpu,
     }
@@ -450,6 +454,7 @@ def main():
     if args.config == 'fhs2048x64':
         return cube_main(args)
     ctx = fhs.default_context(0)
+    ctx.set_autotune(not args.no_autotune)
     t_setup = time.perf_counter()
     facets = None
     if args.config == 'fhs1024':
@@ -722,6 +727,7 @@ def main():
                      'survey_formula_bytes_per_launch': None if dr else
                      int(sum(8 * g.nnz + 12 * g.M + 8 * meas.nufft.No for g in meas.G))},
         'kernels': kernels,
+        'tuned_launch_shapes': ctx.autotune_report(),
         'solve': solve,
         'cpu_baseline': cpu,
     }

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -69,6 +70,7 @@ struct fhs_ctx {
   fhs::Workspace ws;
   // live kernel timing (HIP events on the context stream)
   bool timing = false;
+  bool autotune = false;               // time the launch-shape candidates of tunable kernels on first use
   std::vector<fhs::KernelTimer> pending;
   std::map<std::string, std::pair<double, long>> totals;  // ms, count
   std::vector<hipEvent_t> event_pool;
@@ -92,6 +94,66 @@ struct Scope {
     if (on) ctx->end(t);
   }
 };
+
+// ---- autotuner ---------------------------------------------------------
+// Launch shapes that change only how the work is spread over workgroups,
+// never the arithmetic of an output (rows per workgroup of an FFT pass, rows
+// staged per chunk of a gridding tile), are chosen per device and per
+// problem shape: with autotuning on, the first call of a tunable kernel
+// outside graph capture times every candidate three times with HIP events
+// on the context stream (the kernels are pure functions of their inputs, so
+// re-running them leaves the same outputs) and keeps the fastest; the choice
+// is then reused by every context on that device.  With autotuning off, or
+// under capture before a choice exists, the default candidate runs.
+int tuned_choice(int device, const std::string& key);   // -1: not tuned yet
+void set_tuned(int device, const std::string& key, int choice, const std::string& label);
+std::string tuned_report(int device);
+
+template <class F>
+int tune_launch(fhs_ctx* ctx, const std::string& key, int ncand, int dflt, const char* const* labels, F&& launch) {
+  if (ncand <= 1 || !ctx->autotune) {
+    launch(dflt);
+    return dflt;
+  }
+  const int known = tuned_choice(ctx->device, key);
+  if (known >= 0) {
+    launch(known);
+    return known;
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  FHS_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+  if (cap != hipStreamCaptureStatusNone) {
+    launch(dflt);
+    return dflt;
+  }
+  constexpr int ROUNDS = 3;
+  std::vector<hipEvent_t> ev(2 * ROUNDS * ncand);
+  for (auto& e : ev) FHS_HIP(hipEventCreate(&e));
+  for (int r = 0; r < ROUNDS; ++r)
+    for (int c = 0; c < ncand; ++c) {
+      FHS_HIP(hipEventRecord(ev[2 * (r * ncand + c)], ctx->stream));
+      launch(c);
+      FHS_HIP(hipEventRecord(ev[2 * (r * ncand + c) + 1], ctx->stream));
+    }
+  FHS_HIP(hipEventSynchronize(ev.back()));
+  int best = dflt;
+  float best_ms = 0.f;
+  for (int c = 0; c < ncand; ++c) {
+    float m = 0.f;
+    for (int r = 0; r < ROUNDS; ++r) {
+      float t = 0.f;
+      FHS_HIP(hipEventElapsedTime(&t, ev[2 * (r * ncand + c)], ev[2 * (r * ncand + c) + 1]));
+      m = r == 0 ? t : std::min(m, t);
+    }
+    if (c == 0 || m < best_ms) {
+      best_ms = m;
+      best = c;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  set_tuned(ctx->device, key, best, labels ? labels[best] : std::to_string(best));
+  return best;
+}
 
 // Runs body, converting C++ exceptions to a status code and a message.
 template <class F>

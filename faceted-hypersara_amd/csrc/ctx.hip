@@ -1,8 +1,40 @@
 // Context lifetime, error reporting and live kernel timing.
 #include <cstring>
+#include <mutex>
 
 #include "common.hpp"
 #include "kernels.hpp"
+
+namespace fhs {
+
+// The autotuner's choices, per device and problem key (shared by every
+// context of the process; the mutex covers contexts used from several host
+// threads).
+namespace {
+std::mutex g_tune_mu;
+std::map<std::pair<int, std::string>, std::pair<int, std::string>> g_tuned;
+}  // namespace
+
+int tuned_choice(int device, const std::string& key) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned.find({device, key});
+  return it == g_tuned.end() ? -1 : it->second.first;
+}
+
+void set_tuned(int device, const std::string& key, int choice, const std::string& label) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[{device, key}] = {choice, label};
+}
+
+std::string tuned_report(int device) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::string r;
+  for (const auto& kv : g_tuned)
+    if (kv.first.first == device) r += kv.first.second + "=" + kv.second.second + "\n";
+  return r;
+}
+
+}  // namespace fhs
 
 hipEvent_t fhs_ctx::take_event() {
   if (!event_pool.empty()) {
@@ -89,6 +121,25 @@ int fhs_ctx_set_stream(fhs_ctx* ctx, void* stream) {
 int fhs_ctx_synchronize(fhs_ctx* ctx) {
   if (!ctx) return FHS_ERR_ARG;
   return fhs::guard(ctx, [&] { FHS_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int fhs_ctx_set_autotune(fhs_ctx* ctx, int enable) {
+  if (!ctx) return FHS_ERR_ARG;
+  ctx->autotune = enable != 0;
+  return FHS_OK;
+}
+
+int fhs_autotune_report(fhs_ctx* ctx, char* buf, int cap, int* len) {
+  if (!ctx) return FHS_ERR_ARG;
+  return fhs::guard(ctx, [&] {
+    const std::string r = fhs::tuned_report(ctx->device);
+    if (len) *len = (int)r.size();
+    if (buf && cap > 0) {
+      const int n = std::min<int>(cap - 1, (int)r.size());
+      std::memcpy(buf, r.data(), n);
+      buf[n] = 0;
+    }
+  });
 }
 
 int fhs_ctx_set_timing(fhs_ctx* ctx, int enable) {

@@ -20,6 +20,8 @@
 // odd): measured faster than the register form for these strided passes.
 // Twiddles come from an FP64-computed table exp(-2 pi i k/n).
 #include <cmath>
+#include <string>
+#include <type_traits>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -267,9 +269,9 @@ constexpr int fft_threads() { return B * n / 16 < 64 ? 64 : (B * n / 16 > 1024 ?
 // XCD's L2 by xpass_row0.
 template <int N>
 constexpr int b_strided() { return N >= 2048 ? 4 : (N >= 1024 ? 8 : 16); }
-template <int N>
+template <int N, int B>
 constexpr int nt_strided() {
-  return b_strided<N>() * N / 4 > 1024 ? 1024 : (b_strided<N>() * N / 4 < 64 ? 64 : b_strided<N>() * N / 4);
+  return B * N / 4 > 1024 ? 1024 : (B * N / 4 < 64 ? 64 : B * N / 4);
 }
 
 // x-pass workgroup -> first grid row.  Workgroups are dispatched to the 8
@@ -278,9 +280,8 @@ constexpr int nt_strided() {
 // workgroups share each line of every grid column), workgroups b, b + 8, ...,
 // b + 8 (G - 1) take G adjacent row groups, so the line is fetched into one
 // XCD's L2 instead of G.
-template <int N>
+template <int B>
 __device__ __forceinline__ int xpass_row0() {
-  constexpr int B = b_strided<N>();
   constexpr int G = 16 / B;
   const int b = blockIdx.x;
   if constexpr (G == 1) {
@@ -317,15 +318,14 @@ __global__ __launch_bounds__(nt_y<N>()) void k_fwd_y(const float* __restrict__ x
 }
 
 // (2) forward x-pass: N = Ktx
-template <int N>
-__global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restrict__ T, int Kty,
+template <int N, int B>
+__global__ __launch_bounds__((nt_strided<N, B>())) void k_fwd_x(const float2* __restrict__ T, int Kty,
                                                           int Nx, const float2* __restrict__ tw,
                                                           float2* __restrict__ Fx) {
-  constexpr int B = b_strided<N>();
-  constexpr int NT = nt_strided<N>();
+  constexpr int NT = nt_strided<N, B>();
   __shared__ float2 s[N * B];
   __shared__ float2 tws[TW_LDS<N>];
-  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
+  const int ky0 = xpass_row0<B>(), l = blockIdx.y;
   const float2* twp = stage_twiddles<N, NT, TW_LDS<N>>(tw, tws);
   const float2* Tl = T + (size_t)l * Nx * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
@@ -348,15 +348,14 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x(const float2* __restr
 }
 
 // (1') adjoint x-pass: N = Ktx, keep the first Nx outputs
-template <int N>
-__global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restrict__ g, int Kty,
+template <int N, int B>
+__global__ __launch_bounds__((nt_strided<N, B>())) void k_adj_x(const float2* __restrict__ g, int Kty,
                                                           int Nx, const float2* __restrict__ tw,
                                                           float2* __restrict__ T) {
-  constexpr int B = b_strided<N>();
-  constexpr int NT = nt_strided<N>();
+  constexpr int NT = nt_strided<N, B>();
   __shared__ float2 s[N * B];
   __shared__ float2 tws[TW_LDS<N>];
-  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
+  const int ky0 = xpass_row0<B>(), l = blockIdx.y;
   const float2* twp = stage_twiddles<N, NT, TW_LDS<N>>(tw, tws);
   const float2* gl = g + (size_t)l * N * Kty;
   constexpr int PER = N * B / NT;   // every load issued before the first LDS store
@@ -389,17 +388,16 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_adj_x(const float2* __restr
 // the H-point twiddles being every other entry of the N-point table.
 constexpr int X_SPLIT_MIN = 4096;
 
-template <int N>
-__global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x_split(const float2* __restrict__ T, int Kty,
+template <int N, int B>
+__global__ __launch_bounds__((nt_strided<N, B>())) void k_fwd_x_split(const float2* __restrict__ T, int Kty,
                                                                 int Nx, const float2* __restrict__ tw,
                                                                 float2* __restrict__ Fx) {
   constexpr int H = N / 2;
-  constexpr int B = b_strided<N>();
-  constexpr int NT = nt_strided<N>();
+  constexpr int NT = nt_strided<N, B>();
   static_assert(H * B % NT == 0, "split x-pass: whole rows per thread");
   __shared__ float2 s[H * B];
   __shared__ float2 tws[3 * H / 4];
-  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
+  const int ky0 = xpass_row0<B>(), l = blockIdx.y;
   for (int i = threadIdx.x; i < 3 * H / 4; i += NT) tws[i] = tw[2 * i];
   const float2* Tl = T + (size_t)l * Nx * Kty;
   constexpr int PER = H * B / NT;
@@ -433,17 +431,16 @@ __global__ __launch_bounds__(nt_strided<N>()) void k_fwd_x_split(const float2* _
   }
 }
 
-template <int N>
-__global__ __launch_bounds__(nt_strided<N>()) void k_adj_x_split(const float2* __restrict__ g, int Kty,
+template <int N, int B>
+__global__ __launch_bounds__((nt_strided<N, B>())) void k_adj_x_split(const float2* __restrict__ g, int Kty,
                                                                 int Nx, const float2* __restrict__ tw,
                                                                 float2* __restrict__ T) {
   constexpr int H = N / 2;
-  constexpr int B = b_strided<N>();
-  constexpr int NT = nt_strided<N>();
+  constexpr int NT = nt_strided<N, B>();
   static_assert(H * B % NT == 0, "split x-pass: whole rows per thread");
   __shared__ float2 s[H * B];
   __shared__ float2 tws[3 * H / 4];
-  const int ky0 = xpass_row0<N>(), l = blockIdx.y;
+  const int ky0 = xpass_row0<B>(), l = blockIdx.y;
   for (int i = threadIdx.x; i < 3 * H / 4; i += NT) tws[i] = tw[2 * i];
   const float2* gl = g + (size_t)l * N * Kty;
   constexpr int PER = H * B / NT;
@@ -526,28 +523,100 @@ struct RunFwdY {
     k_fwd_y<N><<<dim3(Nx, L), nt_y<N>(), 0, st>>>(x, p->sy, p->sx, p->Ny, p->Nx, p->twy, T);
   }
 };
+// Rows per x-pass workgroup: the default first, then the autotuner's other
+// candidates (those whose rows fit in LDS).
+template <int N, bool SPLIT>
+struct XRows {
+  static constexpr int n = N >= 4096 ? (SPLIT ? 3 : 2) : N == 2048 ? 3 : N == 1024 ? 3 : N == 512 ? 2 : 1;
+  static constexpr int at(int i) {
+    if (N >= 4096) return i == 0 ? 4 : i == 1 ? 2 : 8;
+    if (N == 2048) return i == 0 ? 4 : i == 1 ? 2 : 8;
+    if (N == 1024) return i == 0 ? 8 : i == 1 ? 4 : 16;
+    if (N == 512) return i == 0 ? 16 : 8;
+    return 16;
+  }
+};
+
+template <int N, bool SPLIT, class F>
+void with_rows(int i, F&& f) {
+  using R = XRows<N, SPLIT>;
+  switch (i) {
+    case 0: f(std::integral_constant<int, R::at(0)>{}); break;
+    case 1: if constexpr (R::n > 1) f(std::integral_constant<int, R::at(1)>{}); break;
+    default: if constexpr (R::n > 2) f(std::integral_constant<int, R::at(2)>{}); break;
+  }
+}
+
+template <int N, bool SPLIT>
+const char* const* xrow_labels() {
+  static const std::string lab[3] = {std::to_string(XRows<N, SPLIT>::at(0)) + " rows",
+                                     std::to_string(XRows<N, SPLIT>::at(1)) + " rows",
+                                     std::to_string(XRows<N, SPLIT>::at(2)) + " rows"};
+  static const char* const p[3] = {lab[0].c_str(), lab[1].c_str(), lab[2].c_str()};
+  return p;
+}
+
+inline std::string xpass_key(const char* k, int N, int Kty, int Nx, int L) {
+  return std::string(k) + "/N" + std::to_string(N) + "xKty" + std::to_string(Kty) + "xNx" + std::to_string(Nx) +
+         "xL" + std::to_string(L);
+}
+
 template <int N>
 struct RunFwdX {
-  static void run(hipStream_t st, int Kty, int L, const float2* T, const fhs_nufft* p, float2* F) {
+  static void run(fhs_ctx* ctx, int Kty, int L, const float2* T, const fhs_nufft* p, float2* F) {
+    const bool split = N >= X_SPLIT_MIN && 2 * p->Nx <= N;
+    auto go = [&](auto split_c) {
+      constexpr bool SP = decltype(split_c)::value;
+      tune_launch(ctx, xpass_key("fft_fwd_x", N, Kty, p->Nx, L), XRows<N, SP>::n, 0, xrow_labels<N, SP>(),
+                  [&](int c) {
+                    with_rows<N, SP>(c, [&](auto b_c) {
+                      constexpr int B = decltype(b_c)::value;
+                      if constexpr (SP)
+                        k_fwd_x_split<N, B><<<dim3(Kty / B, L), nt_strided<N, B>(), 0, ctx->stream>>>(T, Kty, p->Nx,
+                                                                                                    p->twx, F);
+                      else
+                        k_fwd_x<N, B><<<dim3(Kty / B, L), nt_strided<N, B>(), 0, ctx->stream>>>(T, Kty, p->Nx,
+                                                                                              p->twx, F);
+                      check_launch();
+                    });
+                  });
+    };
     if constexpr (N >= X_SPLIT_MIN) {
-      if (2 * p->Nx <= N) {
-        k_fwd_x_split<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(T, Kty, p->Nx, p->twx, F);
+      if (split) {
+        go(std::true_type{});
         return;
       }
     }
-    k_fwd_x<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(T, Kty, p->Nx, p->twx, F);
+    go(std::false_type{});
   }
 };
 template <int N>
 struct RunAdjX {
-  static void run(hipStream_t st, int Kty, int L, const float2* g, const fhs_nufft* p, float2* T) {
+  static void run(fhs_ctx* ctx, int Kty, int L, const float2* g, const fhs_nufft* p, float2* T) {
+    const bool split = N >= X_SPLIT_MIN && 2 * p->Nx <= N;
+    auto go = [&](auto split_c) {
+      constexpr bool SP = decltype(split_c)::value;
+      tune_launch(ctx, xpass_key("fft_adj_x", N, Kty, p->Nx, L), XRows<N, SP>::n, 0, xrow_labels<N, SP>(),
+                  [&](int c) {
+                    with_rows<N, SP>(c, [&](auto b_c) {
+                      constexpr int B = decltype(b_c)::value;
+                      if constexpr (SP)
+                        k_adj_x_split<N, B><<<dim3(Kty / B, L), nt_strided<N, B>(), 0, ctx->stream>>>(g, Kty, p->Nx,
+                                                                                                    p->twx, T);
+                      else
+                        k_adj_x<N, B><<<dim3(Kty / B, L), nt_strided<N, B>(), 0, ctx->stream>>>(g, Kty, p->Nx,
+                                                                                              p->twx, T);
+                      check_launch();
+                    });
+                  });
+    };
     if constexpr (N >= X_SPLIT_MIN) {
-      if (2 * p->Nx <= N) {
-        k_adj_x_split<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(g, Kty, p->Nx, p->twx, T);
+      if (split) {
+        go(std::true_type{});
         return;
       }
     }
-    k_adj_x<N><<<dim3(Kty / b_strided<N>(), L), nt_strided<N>(), 0, st>>>(g, Kty, p->Nx, p->twx, T);
+    go(std::false_type{});
   }
 };
 template <int N>
@@ -582,7 +651,7 @@ void nufft_forward(fhs_ctx* ctx, const fhs_nufft* p, const float* x, int L, floa
   }
   {
     Scope sc(ctx, "fft_fwd_x");
-    dispatch_pow2<RunFwdX>(p->Ktx, ctx->stream, p->Kty, L, (const float2*)T, p, Fx);
+    dispatch_pow2<RunFwdX>(p->Ktx, ctx, p->Kty, L, (const float2*)T, p, Fx);
     check_launch();
   }
 }
@@ -592,7 +661,7 @@ void nufft_adjoint(fhs_ctx* ctx, const fhs_nufft* p, const float2* g, int L, con
   float2* T = (float2*)ctx->ws.get("nufft_T", (size_t)L * p->Nx * p->Kty * sizeof(float2));
   {
     Scope sc(ctx, "fft_adj_x");
-    dispatch_pow2<RunAdjX>(p->Ktx, ctx->stream, p->Kty, L, g, p, T);
+    dispatch_pow2<RunAdjX>(p->Ktx, ctx, p->Kty, L, g, p, T);
     check_launch();
   }
   {

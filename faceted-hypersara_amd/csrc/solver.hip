@@ -328,6 +328,8 @@ void check_published(fhs_pdfb* P) {
 void begin_calls(fhs_pdfb* P) {
   P->own.timing = P->ctx->timing;
   P->side.timing = P->ctx->timing;
+  P->own.autotune = P->ctx->autotune;
+  P->side.autotune = P->ctx->autotune;
   P->low.timing = P->ctx->timing;
 }
 

@@ -29,6 +29,8 @@ PROTOTYPES = {
     'fhs_ctx_set_stream': [c_void_p, c_void_p],
     'fhs_ctx_synchronize': [c_void_p],
     'fhs_ctx_set_timing': [c_void_p, c_int],
+    'fhs_ctx_set_autotune': [c_void_p, c_int],
+    'fhs_autotune_report': [c_void_p, C.c_char_p, c_int, c_int_p],
     'fhs_ctx_timing_report': [c_void_p, C.c_char_p, c_int, c_double_p, c_long_p, c_int, c_int_p],
     'fhs_ctx_timing_reset': [c_void_p],
     'fhs_gmat_create_csr_host': [c_void_p, c_i64, c_i64, c_i64_p, c_i64_p, c_float_p, C.POINTER(c_void_p)],

@@ -67,6 +67,23 @@ class Context:
         keys = names.value.decode().split('\n')[:n.value]
         return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
 
+    def set_autotune(self, on):
+        """Launch-shape autotuning (``fhs_ctx_set_autotune``): results are
+        bit-identical either way."""
+        self.check(lib().fhs_ctx_set_autotune(self.h, int(bool(on))), 'set_autotune')
+
+    def autotune_report(self):
+        """{kernel/shape: choice} of the launch shapes tuned on this device."""
+        n = C.c_int()
+        self.check(lib().fhs_autotune_report(self.h, None, 0, C.byref(n)), 'autotune_report')
+        buf = C.create_string_buffer(n.value + 1)
+        self.check(lib().fhs_autotune_report(self.h, buf, n.value + 1, C.byref(n)), 'autotune_report')
+        out = {}
+        for line in buf.value.decode().splitlines():
+            k, _, v = line.partition('=')
+            out[k] = v
+        return out
+
     def timing_reset(self):
         self.check(lib().fhs_ctx_timing_reset(self.h), 'timing_reset')
 

@@ -48,6 +48,15 @@ const char* fhs_last_error(const fhs_ctx* ctx);
 /* Use an existing HIP stream (hipStream_t passed as void*); NULL = null stream. */
 int fhs_ctx_set_stream(fhs_ctx* ctx, void* stream);
 int fhs_ctx_synchronize(fhs_ctx* ctx);
+/* Autotuning of launch shapes that never change the arithmetic of an output
+ * (rows per workgroup of the FFT x-passes, rows staged per chunk of the
+ * gridding tiles): with enable != 0, the first call of such a kernel outside
+ * graph capture times every candidate with HIP events and keeps the fastest
+ * for that device and problem shape; results are bit-identical to the
+ * defaults.  No reference counterpart (the reference's MATLAB has no launch
+ * shapes).  The report lists "kernel/shape=choice" lines (len: its length). */
+int fhs_ctx_set_autotune(fhs_ctx* ctx, int enable);
+int fhs_autotune_report(fhs_ctx* ctx, char* buf, int cap, int* len);
 /* Live per-kernel timing with HIP events on the context stream. */
 int fhs_ctx_set_timing(fhs_ctx* ctx, int enable);
 /* Copies up to cap entries: names as "name\n" concatenated into names_buf
