@@ -639,6 +639,252 @@ float2* twiddles(int n) {
   return d;
 }
 
+// ---- grids that are not powers of two: runtime-length mixed radix ---------
+// Grid sizes N = 2^a 3^b 5^c (an oversampled 2560-pixel axis is 5120 =
+// 16 16 4 5) run the same register Stockham stages with the length known at
+// run time: radix 16 while N/NS divides by 16, then the remaining power of
+// two, then radix 5 and 3 stages (dft3 / dft5 below).  Each thread holds up
+// to ceil(16 / R) butterflies of a radix-R stage; the x-passes use the same
+// core on B rows per workgroup (no split form).
+__device__ __forceinline__ int fft_radix_rt(int rem) {
+  if (rem % 16 == 0) return 16;
+  const int p2 = rem & -rem;
+  if (p2 > 1) return p2;
+  return rem % 5 == 0 ? 5 : 3;
+}
+
+template <bool INV>
+__device__ __forceinline__ float2 mul_i(float2 z, float s) {   // z * (-i s) forward, z * (+i s) inverse
+  return INV ? make_float2(-z.y * s, z.x * s) : make_float2(z.y * s, -z.x * s);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft3(float2* x) {
+  const float c = -0.5f, s = 0.866025403784438647f;   // cos, sin of 2 pi / 3
+  const float2 t1 = cadd(x[1], x[2]);
+  const float2 t2 = mul_i<INV>(csub(x[1], x[2]), s);
+  const float2 u = make_float2(x[0].x + c * t1.x, x[0].y + c * t1.y);
+  x[0] = cadd(x[0], t1);
+  x[1] = cadd(u, t2);
+  x[2] = csub(u, t2);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2* x) {
+  const float c1 = 0.309016994374947424f, s1 = 0.951056516295153572f;    // 2 pi / 5
+  const float c2 = -0.809016994374947424f, s2 = 0.587785252292473129f;   // 4 pi / 5
+  const float2 a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+  const float2 a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+  const float2 u1 = make_float2(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+  const float2 u2 = make_float2(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+  const float2 t1 = mul_i<INV>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), 1.f);
+  const float2 t2 = mul_i<INV>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), 1.f);
+  x[0] = cadd(x[0], cadd(a1, a2));
+  x[1] = cadd(u1, t1);
+  x[2] = cadd(u2, t2);
+  x[3] = csub(u2, t2);
+  x[4] = csub(u1, t1);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2* v) {
+  if constexpr (R == 3)
+    dft3<INV>(v);
+  else if constexpr (R == 5)
+    dft5<INV>(v);
+  else
+    dft_reg<R, INV>(v);
+}
+
+template <int R, bool INV, class Load, class Store>
+__device__ __forceinline__ void fft_stage_rt(float2* s, const float2* __restrict__ tw, int N, int NS, int B, int NT,
+                                             bool first, bool last, Load& load, Store& store) {
+  constexpr int PMAX = (16 + R - 1) / R;   // butterflies per thread (NT >= B N / 16)
+  const int NBF = B * N / R;
+  const int TS = N / (NS * R);
+  float2 v[PMAX][R];
+#pragma unroll
+  for (int q = 0; q < PMAX; ++q) {
+    const int u = threadIdx.x + q * NT;
+    if (u < NBF) {
+      const int b = u % B, j = u / B;
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int n = j + r * (N / R);
+        v[q][r] = first ? load(q, r, b, n) : s[fft_pad(n) * B + b];
+      }
+      if (k) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * TS];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft_any<R, INV>(v[q]);
+    }
+  }
+  if (!first && !last) __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PMAX; ++q) {
+    const int u = threadIdx.x + q * NT;
+    if (u < NBF) {
+      const int b = u % B, j = u / B;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int n = base + r * NS;
+        if (last)
+          store(q, r, b, n, v[q][r]);
+        else
+          s[fft_pad(n) * B + b] = v[q][r];
+      }
+    }
+  }
+  if (!last) __syncthreads();
+}
+
+template <bool INV, class Load, class Store>
+__device__ void fft_rt(float2* s, const float2* __restrict__ tw, int N, int B, int NT, Load& load, Store& store) {
+  int ns = 1;
+  bool first = true;
+  while (ns < N) {
+    const int R = fft_radix_rt(N / ns);
+    const bool last = ns * R == N;
+    switch (R) {
+      case 16: fft_stage_rt<16, INV>(s, tw, N, ns, B, NT, first, last, load, store); break;
+      case 8: fft_stage_rt<8, INV>(s, tw, N, ns, B, NT, first, last, load, store); break;
+      case 4: fft_stage_rt<4, INV>(s, tw, N, ns, B, NT, first, last, load, store); break;
+      case 2: fft_stage_rt<2, INV>(s, tw, N, ns, B, NT, first, last, load, store); break;
+      case 5: fft_stage_rt<5, INV>(s, tw, N, ns, B, NT, first, last, load, store); break;
+      default: fft_stage_rt<3, INV>(s, tw, N, ns, B, NT, first, last, load, store); break;
+    }
+    ns *= R;
+    first = false;
+  }
+}
+
+// threads and LDS bytes of a runtime-length workgroup of B transforms
+inline int fft_rt_threads(int N, int B) {
+  const int t = (B * N + 15) / 16;
+  return std::max(64, (t + 63) / 64 * 64);
+}
+inline size_t fft_rt_lds(int N, int B) { return ((size_t)fft_pad(N) * B + B) * sizeof(float2); }
+// rows per runtime x-pass workgroup: 4, 2 or 1, dividing the other axis, with
+// one radix-16 butterfly per thread (B N <= 16384) and the rows in LDS
+inline int fft_rt_rows(int N, int other) {
+  for (int B : {4, 2, 1})
+    if (other % B == 0 && B * N <= 16384 && fft_rt_lds(N, B) <= 150 * 1024) return B;
+  return 1;
+}
+
+__global__ __launch_bounds__(1024) void k_fwd_y_rt(const float* __restrict__ x, const float* __restrict__ sy,
+                                                   const float* __restrict__ sx, int Ny, int Nx, int N,
+                                                   const float2* __restrict__ tw, float2* __restrict__ T) {
+  extern __shared__ __align__(16) float2 s_rt[];
+  const int ix = blockIdx.x, l = blockIdx.y;
+  const float* __restrict__ xc = x + ((size_t)l * Nx + ix) * Ny;
+  const float scx = sx[ix];
+  float2* __restrict__ out = T + ((size_t)l * Nx + ix) * N;
+  auto load = [&](int, int, int, int n) {
+    return n < Ny ? make_float2(xc[n] * sy[n] * scx, 0.f) : make_float2(0.f, 0.f);
+  };
+  auto store = [&](int, int, int, int n, float2 v) { out[n] = v; };
+  fft_rt<false>(s_rt, tw, N, 1, blockDim.x, load, store);
+}
+
+__global__ __launch_bounds__(1024) void k_adj_y_rt(const float2* __restrict__ T, const float* __restrict__ sy,
+                                                   const float* __restrict__ sx, int Ny, int Nx, int N,
+                                                   const float2* __restrict__ tw, const float* __restrict__ scale,
+                                                   float* __restrict__ out) {
+  extern __shared__ __align__(16) float2 s_rt[];
+  const int ix = blockIdx.x, l = blockIdx.y;
+  const float2* __restrict__ Tc = T + ((size_t)l * Nx + ix) * N;
+  const float sc = (scale ? scale[l] : 1.f) * sx[ix];
+  float* __restrict__ o = out + ((size_t)l * Nx + ix) * Ny;
+  auto load = [&](int, int, int, int n) { return Tc[n]; };
+  auto store = [&](int, int, int, int n, float2 v) {
+    if (n < Ny) o[n] = sc * sy[n] * v.x;
+  };
+  fft_rt<true>(s_rt, tw, N, 1, blockDim.x, load, store);
+}
+
+__global__ __launch_bounds__(1024) void k_fwd_x_rt(const float2* __restrict__ T, int Kty, int Nx, int N, int B,
+                                                   const float2* __restrict__ tw, float2* __restrict__ Fx) {
+  extern __shared__ __align__(16) float2 s_rt[];
+  const int ky0 = blockIdx.x * B, l = blockIdx.y;
+  const float2* __restrict__ Tl = T + (size_t)l * Nx * Kty;
+  float2* __restrict__ F = Fx + (size_t)l * N * Kty;
+  auto load = [&](int, int, int b, int n) {
+    return n < Nx ? Tl[(size_t)n * Kty + ky0 + b] : make_float2(0.f, 0.f);
+  };
+  auto store = [&](int, int, int b, int n, float2 v) { F[(size_t)n * Kty + ky0 + b] = v; };
+  fft_rt<false>(s_rt, tw, N, B, blockDim.x, load, store);
+}
+
+__global__ __launch_bounds__(1024) void k_adj_x_rt(const float2* __restrict__ g, int Kty, int Nx, int N, int B,
+                                                   const float2* __restrict__ tw, float2* __restrict__ T) {
+  extern __shared__ __align__(16) float2 s_rt[];
+  const int ky0 = blockIdx.x * B, l = blockIdx.y;
+  const float2* __restrict__ gl = g + (size_t)l * N * Kty;
+  float2* __restrict__ Tl = T + (size_t)l * Nx * Kty;
+  auto load = [&](int, int, int b, int n) { return gl[(size_t)n * Kty + ky0 + b]; };
+  auto store = [&](int, int, int b, int n, float2 v) {
+    if (n < Nx) Tl[(size_t)n * Kty + ky0 + b] = v;
+  };
+  fft_rt<true>(s_rt, tw, N, B, blockDim.x, load, store);
+}
+
+inline bool is_pow2_grid(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; }
+
+void fwd_y_any(fhs_ctx* ctx, const fhs_nufft* p, int L, const float* x, float2* T) {
+  if (is_pow2_grid(p->Kty)) {
+    dispatch_pow2<RunFwdY>(p->Kty, ctx->stream, p->Nx, L, x, p, T);
+    return;
+  }
+  const int nt = fft_rt_threads(p->Kty, 1);
+  const size_t lds = fft_rt_lds(p->Kty, 1);
+  allow_dynamic_lds((const void*)k_fwd_y_rt, lds);
+  k_fwd_y_rt<<<dim3(p->Nx, L), nt, lds, ctx->stream>>>(x, p->sy, p->sx, p->Ny, p->Nx, p->Kty, p->twy, T);
+}
+
+void adj_y_any(fhs_ctx* ctx, const fhs_nufft* p, int L, const float2* T, const float* scale, float* out) {
+  if (is_pow2_grid(p->Kty)) {
+    dispatch_pow2<RunAdjY>(p->Kty, ctx->stream, p->Nx, L, T, p, scale, out);
+    return;
+  }
+  const int nt = fft_rt_threads(p->Kty, 1);
+  const size_t lds = fft_rt_lds(p->Kty, 1);
+  allow_dynamic_lds((const void*)k_adj_y_rt, lds);
+  k_adj_y_rt<<<dim3(p->Nx, L), nt, lds, ctx->stream>>>(T, p->sy, p->sx, p->Ny, p->Nx, p->Kty, p->twy, scale, out);
+}
+
+void fwd_x_any(fhs_ctx* ctx, const fhs_nufft* p, int L, const float2* T, float2* F) {
+  if (is_pow2_grid(p->Ktx)) {
+    dispatch_pow2<RunFwdX>(p->Ktx, ctx, p->Kty, L, T, p, F);
+    return;
+  }
+  const int B = fft_rt_rows(p->Ktx, p->Kty);
+  const int nt = fft_rt_threads(p->Ktx, B);
+  const size_t lds = fft_rt_lds(p->Ktx, B);
+  allow_dynamic_lds((const void*)k_fwd_x_rt, lds);
+  k_fwd_x_rt<<<dim3(p->Kty / B, L), nt, lds, ctx->stream>>>(T, p->Kty, p->Nx, p->Ktx, B, p->twx, F);
+}
+
+void adj_x_any(fhs_ctx* ctx, const fhs_nufft* p, int L, const float2* g, float2* T) {
+  if (is_pow2_grid(p->Ktx)) {
+    dispatch_pow2<RunAdjX>(p->Ktx, ctx, p->Kty, L, g, p, T);
+    return;
+  }
+  const int B = fft_rt_rows(p->Ktx, p->Kty);
+  const int nt = fft_rt_threads(p->Ktx, B);
+  const size_t lds = fft_rt_lds(p->Ktx, B);
+  allow_dynamic_lds((const void*)k_adj_x_rt, lds);
+  k_adj_x_rt<<<dim3(p->Kty / B, L), nt, lds, ctx->stream>>>(g, p->Kty, p->Nx, p->Ktx, B, p->twx, T);
+}
+
 }  // namespace
 
 // Used by the fused data-fidelity step.
@@ -646,12 +892,12 @@ void nufft_forward(fhs_ctx* ctx, const fhs_nufft* p, const float* x, int L, floa
   float2* T = (float2*)ctx->ws.get("nufft_T", (size_t)L * p->Nx * p->Kty * sizeof(float2));
   {
     Scope sc(ctx, "fft_fwd_y");
-    dispatch_pow2<RunFwdY>(p->Kty, ctx->stream, p->Nx, L, x, p, T);
+    fwd_y_any(ctx, p, L, x, T);
     check_launch();
   }
   {
     Scope sc(ctx, "fft_fwd_x");
-    dispatch_pow2<RunFwdX>(p->Ktx, ctx, p->Kty, L, (const float2*)T, p, Fx);
+    fwd_x_any(ctx, p, L, (const float2*)T, Fx);
     check_launch();
   }
 }
@@ -661,12 +907,12 @@ void nufft_adjoint(fhs_ctx* ctx, const fhs_nufft* p, const float2* g, int L, con
   float2* T = (float2*)ctx->ws.get("nufft_T", (size_t)L * p->Nx * p->Kty * sizeof(float2));
   {
     Scope sc(ctx, "fft_adj_x");
-    dispatch_pow2<RunAdjX>(p->Ktx, ctx, p->Kty, L, g, p, T);
+    adj_x_any(ctx, p, L, g, T);
     check_launch();
   }
   {
     Scope sc(ctx, "fft_adj_y");
-    dispatch_pow2<RunAdjY>(p->Kty, ctx->stream, p->Nx, L, (const float2*)T, p, scale_dev, out);
+    adj_y_any(ctx, p, L, (const float2*)T, scale_dev, out);
     check_launch();
   }
 }
@@ -682,8 +928,13 @@ int fhs_nufft_create_host(fhs_ctx* ctx, int Ny, int Nx, int Kty, int Ktx, const 
   if (!ctx || !out) return FHS_ERR_ARG;
   *out = nullptr;
   return guard(ctx, [&] {
-    auto pow2 = [](int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; };
-    FHS_CHECK(pow2(Kty) && pow2(Ktx), "nufft_create: Kty, Ktx must be powers of two in [16, 4096]");
+    auto smooth = [](int n) {   // 2^a 3^b 5^c in [16, 8192]
+      if (n < 16 || n > 8192) return false;
+      for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+      return n == 1;
+    };
+    FHS_CHECK(smooth(Kty) && smooth(Ktx), "nufft_create: Kty, Ktx must be of the form 2^a 3^b 5^c in [16, 8192]");
     FHS_CHECK(Ny >= 1 && Nx >= 1 && Ny <= Kty && Nx <= Ktx, "nufft_create: image larger than grid");
     FHS_HIP(hipSetDevice(ctx->device));
     fhs_nufft* p = new fhs_nufft();

@@ -132,8 +132,10 @@ int fhs_holographic_grid(fhs_ctx* ctx, const fhs_gmat* G, const fhs_gmat* H, con
 
 /* ------------------------------------------------- scaled Fourier (A) -- */
 /* Plan for A = fft2(pad(sn .* x)) with sn = sy * sx' (real scaling factors
- * of the NUFFT, host arrays sy[Ny], sx[Nx]); grid Kty x Ktx, powers of two
- * in [16, 4096], Kty >= Ny, Ktx >= Nx. */
+ * of the NUFFT, host arrays sy[Ny], sx[Nx]); grid Kty x Ktx of the form
+ * 2^a 3^b 5^c in [16, 8192] (powers of two up to 4096 take the tuned
+ * fixed-length kernels, other sizes a runtime mixed-radix path), Kty >= Ny,
+ * Ktx >= Nx. */
 int fhs_nufft_create_host(fhs_ctx* ctx, int Ny, int Nx, int Kty, int Ktx,
                           const double* sy, const double* sx, fhs_nufft** out);
 int fhs_nufft_destroy(fhs_nufft* p);

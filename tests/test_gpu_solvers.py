@@ -25,11 +25,13 @@ def oracle_problem(prob, meas):
                          g_scale=[1.0 / s for s in prob.sigma_noise])
 
 
-@pytest.mark.parametrize('precond', [True, False])
-def test_hypersara_matches_oracle(ctx, precond):
+# 40 x 48 has an 80 x 96 grid (16 5 and 32 3): the mixed-radix FFT path and
+# the untiled G layouts
+@pytest.mark.parametrize('precond,Ny,Nx', [(True, 32, 32), (False, 32, 32), (True, 40, 48)])
+def test_hypersara_matches_oracle(ctx, precond, Ny, Nx):
     import fhs
     from fhs import solvers, synth
-    prob = synth.make_problem(32, 32, 4, 1500, 2, seed=3, n_sources=20)
+    prob = synth.make_problem(Ny, Nx, 4, 1500, 2, seed=3, n_sources=20)
     meas = solvers.Measurement(ctx, prob, precond=precond)
     son = meas.op_norm(False)
     nu2 = meas.op_norm(precond)
@@ -37,7 +39,7 @@ def test_hypersara_matches_oracle(ctx, precond):
     param = {'squared_operator_norm': son, 'nu2': nu2}
     x_gpu, info = solvers.hyper_sara(meas, param, X0=prob.X0, max_total_iter=K)
     op = oracle_problem(prob, meas)
-    sig, sig_bar = solvers.noise_levels_hs(32, 32, 4, np.ones(4), son)
+    sig, sig_bar = solvers.noise_levels_hs(Ny, Nx, 4, np.ones(4), son)
     pr = dict(solvers.DEFAULT_PARAM)
     pr.update({'nu2': nu2, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar,
                'alpha': 1.0})
