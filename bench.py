@@ -63,6 +63,8 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--gram', choices=('fp64', 'bf16x3', 'bf16x2'), default='fp64',
+                    help='nuclear-norm Gram precision of the throughput engine')
     ap.add_argument('--no-autotune', action='store_true',
                     help='keep the default launch shapes (the tuned ones give bit-identical results)')
     ap.add_argument('--w-max', type=float, default=None,
@@ -316,7 +318,8 @@ def cube_main(args):
         def grab(t, eng, info, holder=holder):
             holder['eng'] = eng
             return t < args.warmup
-        solvers._solve('fhs', meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1)
+        solvers._solve('fhs', meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1,
+                       gram=args.gram)
         engines.append(holder['eng'])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -509,7 +512,7 @@ def main():
         holder['eng'] = eng
         return t < args.warmup
 
-    solvers._solve(kind, meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1)
+    solvers._solve(kind, meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1, gram=args.gram)
     eng = holder['eng']
     torch.cuda.synchronize()
     t0 = time.perf_counter()

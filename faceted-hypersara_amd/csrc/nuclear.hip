@@ -125,6 +125,99 @@ __global__ __launch_bounds__(GRAM_THREADS) void k_gram(FacetDev F, int L, const 
   partial[((int64_t)q * nchunk + chunk) * NUC_L * NUC_L + threadIdx.x] = s;
 }
 
+// Mixed-precision partial Gram (opt-in, fhs_lowrank_set_gram_precision): the
+// same 1024-pixel chunk partials as k_gram, from BF16 split products on the
+// matrix cores.  z (FP32) is split exactly into BF16 parts, z = h + m + l
+// (SPLIT = 3) or approximately, z ~ h + l (SPLIT = 2, 16 mantissa bits), and
+// Z'Z is accumulated by v_mfma_f32_16x16x32_bf16 from the products h'h,
+// h'm, m'h, m'm, h'l, l'h (SPLIT 3) or h'h, h'l, l'h (SPLIT 2) in FP32 over
+// a wave's 256 pixels of the chunk; the four waves' 16 x 16 sums are added in
+// FP64.
+// Each of the 64 lanes holds channel c = lane % 16 at 8 consecutive pixels
+// (group lane / 16) of the instruction's 32: the A operand (Z' rows) and the
+// B operand (Z columns) are the same register, so the product is Z'Z for
+// any assignment of the 32 pixels to the k index.
+typedef __bf16 gm_bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short gm_u16x8 __attribute__((ext_vector_type(8)));
+typedef float gm_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <int SPLIT>
+__global__ __launch_bounds__(GRAM_THREADS) void k_gram_mfma(FacetDev F, int L, const float* __restrict__ v0,
+                                                            const float* __restrict__ xhat,
+                                                            double* __restrict__ partial, int nchunk) {
+  constexpr int LD = GRAM_TP + 4;   // row stride (floats): 16-byte rows, rows 4 banks apart
+  __shared__ __align__(16) float zs[NUC_L][LD];
+  __shared__ double red[GRAM_THREADS / 64][NUC_L * NUC_L];
+  const int q = blockIdx.y, chunk = blockIdx.x;
+  const int py = F.py[q], P = py * F.px[q];
+  const int64_t N = (int64_t)F.Ny * F.Nx;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, grp = lane >> 4;
+  const int p0 = chunk * GRAM_CHUNK;
+  // tiles of 256 pixels as in k_gram: thread t loads pixel t's 16 channels
+  // (coalesced along pixels), the next tile's loads overlapping this one's
+  // products; wave w multiplies pixels [64 w, 64 w + 64) of the tile
+  float z[NUC_L];
+  auto load = [&](int pb) {
+    const int p = pb + (int)threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NUC_L; ++k) z[k] = z_val(F, q, L, P, py, k, p < P ? p : P, v0, xhat, N);
+  };
+  gm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  load(p0);
+  for (int pb = p0; pb < p0 + GRAM_CHUNK; pb += GRAM_TP) {
+#pragma unroll
+    for (int k = 0; k < NUC_L; ++k) zs[k][threadIdx.x] = z[k];
+    __syncthreads();
+    if (pb + GRAM_TP < p0 + GRAM_CHUNK) load(pb + GRAM_TP);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const float4* src = reinterpret_cast<const float4*>(&zs[c][64 * wave + 32 * s + 8 * grp]);
+      const float4 a0 = src[0], a1 = src[1];
+      const float zz[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+      gm_u16x8 h, m, l;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        h[j] = bf16_rne(zz[j]);
+        const float r = zz[j] - bf16_val(h[j]);   // exact
+        if (SPLIT == 3) {
+          m[j] = bf16_rne(r);
+          l[j] = bf16_rne(r - bf16_val(m[j]));     // r - m exact; l rounds the last bits
+        } else {
+          l[j] = bf16_rne(r);
+        }
+      }
+      const gm_bf16x8 H = __builtin_bit_cast(gm_bf16x8, h), Lo = __builtin_bit_cast(gm_bf16x8, l);
+      if (SPLIT == 3) {
+        const gm_bf16x8 M = __builtin_bit_cast(gm_bf16x8, m);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M, M, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H, Lo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lo, H, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H, M, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M, H, acc, 0, 0, 0);
+      } else {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H, Lo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lo, H, acc, 0, 0, 0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H, H, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D[4 grp + r][c] sits in lane (c, grp), register r
+#pragma unroll
+  for (int r = 0; r < 4; ++r) red[wave][(4 * grp + r) * NUC_L + c] = (double)acc[r];
+  __syncthreads();
+  const int e = threadIdx.x;
+  partial[((int64_t)q * nchunk + chunk) * NUC_L * NUC_L + e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+}
+
 // FP64 sum of the per-chunk partial Grams: one workgroup of 1024 threads per
 // facet; thread t adds the chunks k = t / 256 (mod 4) of entry t % 256 with
 // 4 independent accumulators (loads stay in flight), and the four quarters
@@ -399,6 +492,7 @@ struct fhs_lowrank {
   float* d_M = nullptr;                // [Q][16][16]
   double* d_partial = nullptr;         // [Q][nchunk][256]
   double* d_gram = nullptr;            // [Q][256]
+  int gram_mode = 0;                   // 0 FP64, 3 / 2 BF16 split parts on the matrix cores
 };
 
 namespace fhs {
@@ -406,8 +500,13 @@ namespace fhs {
 static void lowrank_run(fhs_ctx* ctx, fhs_lowrank* R, const float* v0, const float* xhat, int mode) {
   {
     Scope sc(ctx, "nuclear_gram");
-    k_gram<<<dim3(R->nchunk, R->Q), GRAM_THREADS, 0, ctx->stream>>>(R->F, R->L, v0, xhat, R->d_partial,
-                                                                    R->nchunk);
+    const dim3 grid(R->nchunk, R->Q);
+    if (R->gram_mode == 3)
+      k_gram_mfma<3><<<grid, GRAM_THREADS, 0, ctx->stream>>>(R->F, R->L, v0, xhat, R->d_partial, R->nchunk);
+    else if (R->gram_mode == 2)
+      k_gram_mfma<2><<<grid, GRAM_THREADS, 0, ctx->stream>>>(R->F, R->L, v0, xhat, R->d_partial, R->nchunk);
+    else
+      k_gram<<<grid, GRAM_THREADS, 0, ctx->stream>>>(R->F, R->L, v0, xhat, R->d_partial, R->nchunk);
     check_launch();
   }
   {
@@ -565,6 +664,12 @@ int fhs_lowrank_destroy(fhs_lowrank* R) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete R;
+  return FHS_OK;
+}
+
+int fhs_lowrank_set_gram_precision(fhs_lowrank* R, int mode) {
+  if (!R || (mode != 0 && mode != 2 && mode != 3)) return FHS_ERR_ARG;
+  R->gram_mode = mode;
   return FHS_OK;
 }
 

@@ -90,6 +90,7 @@ PROTOTYPES = {
                                 c_int_p, c_float_p, C.POINTER(c_void_p)],
     'fhs_lowrank_destroy': [c_void_p],
     'fhs_lowrank_info': [c_void_p, c_i64_p],
+    'fhs_lowrank_set_gram_precision': [c_void_p, c_int],
     'fhs_update_dual_lowrankness': [c_void_p, c_void_p, c_void_p, c_void_p, c_double_p, c_double,
                                     C.c_float, c_void_p],
     'fhs_lowrank_singular_values': [c_void_p, c_void_p, c_void_p, c_double_p],

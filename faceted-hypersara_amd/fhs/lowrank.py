@@ -79,6 +79,18 @@ class LowRankPrior:
         ctx.check(lib().fhs_lowrank_info(h, C.byref(n)), 'lowrank_info')
         self.v0_size = n.value
 
+    GRAM_MODES = {'fp64': 0, 'bf16x3': 3, 'bf16x2': 2}
+
+    def set_gram_precision(self, mode):
+        """'fp64' (default): FP64 Gram of exact products; 'bf16x3' / 'bf16x2':
+        BF16 split parts (hi + mid + lo / hi + lo) multiplied on the matrix
+        cores with FP32 sums (``fhs_lowrank_set_gram_precision``)."""
+        if mode not in self.GRAM_MODES:
+            raise ValueError(f'gram precision must be one of {sorted(self.GRAM_MODES)}')
+        self.ctx.check(lib().fhs_lowrank_set_gram_precision(self.h, self.GRAM_MODES[mode]),
+                       'lowrank_set_gram_precision')
+        self.gram = mode
+
     def zeros_v0(self):
         return torch.zeros(self.v0_size, dtype=torch.float32, device=self.ctx.device)
 

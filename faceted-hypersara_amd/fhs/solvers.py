@@ -345,7 +345,7 @@ def snr_db(X0, x):
 
 def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, graph=True,
            max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True, stop_snr=None,
-           form='engine'):
+           form='engine', gram='fp64'):
     """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'.
 
     With a nonzero ``x_init`` the reference derives the first weights from it
@@ -358,7 +358,9 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     per iteration); ``stop_snr`` (with X0) ends the solve at the first
     iteration whose SNR reaches it.  ``form='dropin'`` runs every iteration
     as the sequence of reference-named entry points instead
-    (``PdfbEngine.dropin_iterate``; SNR by a separate reduction)."""
+    (``PdfbEngine.dropin_iterate``; SNR by a separate reduction).
+    ``gram`` selects the nuclear-norm Gram precision
+    (``LowRankPrior.set_gram_precision``: 'fp64', 'bf16x3', 'bf16x2')."""
     prm = dict(DEFAULT_PARAM)
     prm.update(param or {})
     ctx = meas.ctx
@@ -369,6 +371,8 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
         lowrank = LowRankPrior(ctx, Ny, Nx, L)
     elif kind == 'fhs':
         lowrank = LowRankPrior(ctx, Ny, Nx, L, facets=facets)
+    if lowrank is not None and gram != 'fp64':
+        lowrank.set_gram_precision(gram)
     if x_init is None:
         xi = None
     elif isinstance(x_init, torch.Tensor):          # (L, Nx, Ny) cube already on the GPU

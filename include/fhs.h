@@ -336,6 +336,14 @@ int fhs_lowrank_create_host(fhs_ctx* ctx, int Ny, int Nx, int L, int Qy, int Qx,
                             fhs_lowrank** out);
 int fhs_lowrank_destroy(fhs_lowrank* R);
 int fhs_lowrank_info(const fhs_lowrank* R, int64_t* v0_size);
+/* Precision of the Gram Z'Z inside the nuclear-norm steps (no reference
+ * counterpart: the reference takes an FP64 SVD of Z).  mode 0 (default):
+ * FP64 sums of exact products.  3: BF16 hi + mid + lo parts of z (exact)
+ * multiplied pairwise on the matrix cores, FP32 sums over 256 pixels, FP64
+ * across them.  2: BF16 hi + lo parts (16 mantissa bits).  The eigen-
+ * decomposition stays FP64.  A PDFB engine records its kernels at its first
+ * captured iteration: set the mode before that. */
+int fhs_lowrank_set_gram_precision(fhs_lowrank* R, int mode);
 /* [v0, g0] = fhs_update_dual_lowrankness(v0, xhat, apodization_window,
  *   weights0, beta0)   src/fhs/fhs_update_dual_lowrankness.m:40-45, and
  * hs_update_dual_lowrankness_serial.m:5-10 for one facet.  g0 =

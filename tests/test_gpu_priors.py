@@ -138,3 +138,36 @@ def test_faceted_lowrank_step(ctx, Ny, Nx, Qy, Qx, frac, L):
         assert rel(cube_from_dev(g0d), g0) < TOL, it
     svd_gpu = R.singular_values(cube_to_dev(xhat))
     assert rel(svd_gpu, facet_singular_values(xhat, geom)) < TOL
+
+
+@pytest.mark.parametrize('mode,tol_sv,tol_prox', [('bf16x3', 1e-6, 1e-3), ('bf16x2', 1e-4, 3e-2)])
+@pytest.mark.parametrize('Ny,Nx,Qy,Qx,L', [(256, 256, 1, 1, 16), (192, 160, 2, 2, 11)])
+def test_mixed_precision_gram(ctx, mode, tol_sv, tol_prox, Ny, Nx, Qy, Qx, L):
+    """The BF16 split Gram on the matrix cores against the FP64 oracle: the
+    singular values and the low-rank prox of a near-low-rank cube, with the
+    errors printed (the mixed form is judged on the final SNR of a solve; the
+    FP64 Gram meets the 1e-5 operator tolerance)."""
+    import fhs
+    rng = np.random.default_rng(L)
+    geom = facet_geometry(Ny, Nx, Qy, Qx, (0.25, 0.25))
+    Q = Qy * Qx
+    base = rng.random((Ny, Nx, 3)) @ rng.random((3, L))
+    xhat = (base + 0.05 * rng.normal(size=(Ny, Nx, L))).astype(np.float32).astype(np.float64)
+    sv_ref = facet_singular_values(xhat, geom)
+    w0 = np.tile(np.linspace(1.0, 0.5, L), (Q, 1))
+    beta = float(np.median(sv_ref))
+    v0s = [np.zeros((geom['py'][q] * geom['px'][q], L)) for q in range(Q)]
+    v0s, g0 = faceted_lowrank_step(v0s, xhat, geom, w0, beta, 0.5)
+    ref = np.concatenate([v.T.ravel() for v in v0s])
+    errs = {}
+    for m in ('fp64', mode):
+        R = fhs.LowRankPrior(ctx, Ny, Nx, L, facets=geom)
+        R.set_gram_precision(m)
+        sv = R.singular_values(cube_to_dev(xhat))
+        v0d, g0d = R.update_dual(R.zeros_v0(), cube_to_dev(xhat), w0, beta, 0.5)
+        errs[m] = (rel(sv, sv_ref), rel(v0d.cpu().numpy(), ref), rel(cube_from_dev(g0d), g0))
+    print(f'{mode} {Ny}x{Nx}x{L} Q={Q}: sv err {errs[mode][0]:.2e} prox err {errs[mode][1]:.2e} '
+          f'(fp64 Gram: {errs["fp64"][0]:.2e} / {errs["fp64"][1]:.2e})')
+    assert errs['fp64'][0] < TOL and errs['fp64'][1] < TOL
+    assert errs[mode][0] < tol_sv, errs[mode]
+    assert errs[mode][1] < tol_prox and errs[mode][2] < tol_prox, errs[mode]
