@@ -97,6 +97,7 @@ class Problem:
     y: list = field(default_factory=list)          # whitened visibilities per channel
     sigma_noise: list = field(default_factory=list)
     epsilon: list = field(default_factory=list)    # per channel: list of block bounds
+    datasets: list = field(default_factory=list)   # per channel: rows of each data set (in order)
 
     @property
     def M(self):
@@ -104,14 +105,18 @@ class Problem:
 
 
 def make_problem(Ny, Nx, L, M_base, nblocks, seed=0, flag_range=(0.1, 0.3), input_snr=40.0,
-                 dnu=0.05, n_sources=200, w_max=None, fov=0.08, nu=None, nu_max=None):
+                 dnu=0.05, n_sources=200, w_max=None, fov=0.08, nu=None, nu_max=None, n_datasets=1):
     """Seeded problem.  ``w_max``: base w uniform in [0, w_max] wavelengths
     (scaled per channel with u and v) and field of view ``fov`` for the
     w-term; ``nu``: the channels' frequencies (default 1 + dnu l) and
     ``nu_max`` the frequency the uv coverage is scaled against (default
     max(nu)) -- one spectral sub-cube of a larger cube takes its share of
     the cube's frequencies and the cube's nu_max, and the same seed gives
-    every sub-cube the same sky, spectral indices and base coverage."""
+    every sub-cube the same sky, spectral indices and base coverage.
+    ``n_datasets``: data sets per channel (the reference's ``nDataSets``),
+    each its own base coverage of M_base points, scaled and flagged per
+    channel and split into ``nblocks`` blocks; a channel's rows are its sets
+    one after the other (``Problem.datasets``: rows per set)."""
     rng = np.random.default_rng(seed)
     x0 = sky_image(rng, Ny, Nx, n_sources=n_sources)
     if L == 1:
@@ -121,21 +126,35 @@ def make_problem(Ny, Nx, L, M_base, nblocks, seed=0, flag_range=(0.1, 0.3), inpu
     nu = nu_
     ub, vb = gaussian_uv(rng, M_base)
     wb = rng.uniform(0.0, w_max, M_base) if w_max else None
-    us, vs, ws, blocks = [], [], [], []
+    sets = [(ub, vb, wb)]
+    for _ in range(n_datasets - 1):
+        u2, v2 = gaussian_uv(rng, M_base)
+        sets.append((u2, v2, rng.uniform(0.0, w_max, M_base) if w_max else None))
+    us, vs, ws, blocks, datasets = [], [], [], [], []
     for l in range(L):
         scale = nu[l] / (nu.max() if nu_max is None else nu_max)
-        if L > 1 and flag_range is not None:
-            frac = rng.uniform(*flag_range)
-            keep = np.sort(rng.permutation(M_base)[:int(round(M_base * (1 - frac)))])
-        else:
-            keep = np.arange(M_base)
-        us.append(ub[keep] * scale)
-        vs.append(vb[keep] * scale)
-        if wb is not None:
-            ws.append(wb[keep] * scale)
-        b = np.linspace(0, len(keep), nblocks + 1).astype(np.int64)
-        blocks.append([int(b[i + 1] - b[i]) for i in range(nblocks)])
-    return Problem(Ny, Nx, L, X0, us, vs, blocks, nu, seed, input_snr, w=ws, fov=fov if ws else 0.0)
+        ul, vl, wl, bl, dl = [], [], [], [], []
+        for ub, vb, wb in sets:
+            if L > 1 and flag_range is not None:
+                frac = rng.uniform(*flag_range)
+                keep = np.sort(rng.permutation(M_base)[:int(round(M_base * (1 - frac)))])
+            else:
+                keep = np.arange(M_base)
+            ul.append(ub[keep] * scale)
+            vl.append(vb[keep] * scale)
+            if wb is not None:
+                wl.append(wb[keep] * scale)
+            b = np.linspace(0, len(keep), nblocks + 1).astype(np.int64)
+            bl += [int(b[i + 1] - b[i]) for i in range(nblocks)]
+            dl.append(len(keep))
+        us.append(np.concatenate(ul))
+        vs.append(np.concatenate(vl))
+        if wl:
+            ws.append(np.concatenate(wl))
+        blocks.append(bl)
+        datasets.append(dl)
+    return Problem(Ny, Nx, L, X0, us, vs, blocks, nu, seed, input_snr, w=ws, fov=fov if ws else 0.0,
+                   datasets=datasets)
 
 
 def host_visibilities(problem):

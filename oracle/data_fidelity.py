@@ -142,8 +142,8 @@ def compute_residual_images(x, y, A, At, G, W, flag_visibility_gridding=False, L
 
 def holographic_operator(G, y=None, noise=None):
     """Dimensionality-reduced data of one channel,
-    ``lib/operators/util_load_data_gen_dr_measurement_operator.m:160-225``
-    (one data set; the reference sums the same terms over its data sets).
+    ``lib/operators/util_load_data_gen_dr_measurement_operator.m:160-225``,
+    from one data set (``holographic_operator_sets`` with one set).
 
     ``G``: scipy sparse (M visibilities x No grid points), the measurement
     rows with W = the whole grid (the reference's ``G_curr(W_curr, :) =
@@ -163,19 +163,43 @@ def holographic_operator(G, y=None, noise=None):
     small random matrices: ``H + H^H`` equals it on ``W x W``, H is lower
     triangular, ``Lambda^2 diag = 1``, W are the non-empty columns of G.
     """
+    return holographic_operator_sets([G], None if y is None else [y], None if noise is None else [noise])
+
+
+def holographic_operator_sets(Gs, ys=None, noises=None):
+    """The reference's loop over the data sets (and physical channels) of
+    one effective channel, ``util_load_data_gen_dr_measurement_operator.m:
+    112-196``: ``H += tril(G_curr * G_curr')`` (``:196``), ``gridded_y +=
+    G_curr * y`` (``:171``), the noise ``+= G_curr * noise`` (``:176-192``)
+    per set, then W, Lambda, the data and the noise bound of the sums
+    (``:200-225``).  ``Gs``: list of scipy sparse (M_d x No); ``ys`` /
+    ``noises``: lists of M_d vectors or None.  Same returns as
+    ``holographic_operator``.  Pinned (``tests/test_oracle.py``) against
+    ``holographic_operator`` of the row-stacked G (a sum of Grams is the Gram
+    of the stack)."""
     import scipy.sparse as sp
-    Gc = sp.csr_matrix(G).conj().T.tocsr()          # G_curr: No x M
-    Hfull = sp.tril(Gc @ Gc.conj().T).tocsr()
+    No = Gs[0].shape[1]
+    Hfull = sp.csr_matrix((No, No), dtype=np.complex128)
+    yg = np.zeros(No, dtype=np.complex128)
+    gn = np.zeros(No, dtype=np.complex128)
+    for d, G in enumerate(Gs):
+        Gc = sp.csr_matrix(G).conj().T.tocsr()      # G_curr: No x M_d
+        Hfull = Hfull + sp.tril(Gc @ Gc.conj().T)
+        if ys is not None:
+            yg += Gc @ np.asarray(ys[d])
+        if noises is not None:
+            gn += Gc @ np.asarray(noises[d])
+    Hfull = Hfull.tocsr()
     diagH = np.real(Hfull.diagonal())
     W = np.flatnonzero(diagH > 0)
     Lambda = 1.0 / np.sqrt(diagH[W])
     H = Hfull[W][:, W].tocsr()
     H.setdiag(0.5 * H.diagonal())
     H.eliminate_zeros()
-    y_grid = None if y is None else (Gc @ np.asarray(y))[W] * Lambda
+    y_grid = None if ys is None else yg[W] * Lambda
     l2bound = sigma = None
-    if noise is not None:
-        gn = (Gc @ np.asarray(noise))[W] * Lambda
-        l2bound = float(np.linalg.norm(gn))
-        sigma = float(np.std(gn, ddof=1))
+    if noises is not None:
+        g = gn[W] * Lambda
+        l2bound = float(np.linalg.norm(g))
+        sigma = float(np.std(g, ddof=1))
     return H, W, Lambda, y_grid, l2bound, sigma

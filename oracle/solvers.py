@@ -63,18 +63,22 @@ class OracleProblem:
         self.blocks = blocks
         self.dr, self.Lambda = False, None
 
-    def holographic(self, noise):
+    def holographic(self, noise, datasets=None):
         """The DR form (``imaging/imaging.m:619-631``): per channel one block
         holding H, W, Lambda, the gridded data and the l2 bound of the
-        gridded ``noise[l]`` (``data_fidelity.holographic_operator``); no
-        preconditioning.  Returns a new problem."""
-        from .data_fidelity import holographic_operator
+        gridded ``noise[l]`` (``data_fidelity.holographic_operator_sets``);
+        no preconditioning.  ``datasets[l]``: the row ranges (a, b) of the
+        channel's data sets, summed as the reference sums its data sets
+        (default: all rows, one set).  Returns a new problem."""
+        from .data_fidelity import holographic_operator_sets
         import copy
         P = copy.copy(self)
         P.G, P.W, P.y, P.pU, P.epsilon, P.Lambda = [], [], [], [], [], []
         for l, op in enumerate(self.ops):
             y = np.concatenate(self.y[l])
-            H, W, lam, yg, l2, _ = holographic_operator(op.G, y, noise[l])
+            sets = [(0, op.G.shape[0])] if datasets is None else datasets[l]
+            H, W, lam, yg, l2, _ = holographic_operator_sets(
+                [op.G[a:b] for a, b in sets], [y[a:b] for a, b in sets], [noise[l][a:b] for a, b in sets])
             P.G.append([H])
             P.W.append([W])
             P.y.append([yg])

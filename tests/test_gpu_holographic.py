@@ -211,16 +211,31 @@ def _oracle_problem(prob, meas):
                          g_scale=[1.0 / s for s in prob.sigma_noise])
 
 
-def test_dr_operator_and_adjoint(ctx):
+def _dataset_ranges(prob):
+    """Row ranges of every channel's data sets (rows are set after set)."""
+    out = []
+    for d in prob.datasets:
+        b = np.concatenate([[0], np.cumsum(d)])
+        out.append([(int(b[i]), int(b[i + 1])) for i in range(len(d))])
+    return out
+
+
+# n_datasets = 3: each channel's DR data sum three independent coverages, as
+# the reference sums its data sets (util_load_data_gen_dr_measurement_operator.m:
+# 112-196); the oracle sums per set, the GPU builds from the stacked rows
+@pytest.mark.parametrize('n_datasets', [1, 3])
+def test_dr_operator_and_adjoint(ctx, n_datasets):
     """The DR measurement operator Lambda (H + H') A (HS_forward_operator_G.m,
     istril branch) against the oracle's on the oracle's H, and its adjoint by
-    the identity Re<Phi x, y> = <x, Phi' y>."""
+    the identity Re<Phi x, y> = <x, Phi' y>; the gridded data and the l2
+    bounds of the gridded noise."""
     from fhs import solvers, synth
     from fhs.holographic import HolographicMeasurement
-    prob = synth.make_problem(32, 32, 3, 1500, 2, seed=4, n_sources=20)
+    prob = synth.make_problem(32, 32, 3, 1500 // n_datasets, 2, seed=4, n_sources=20, n_datasets=n_datasets)
     meas = solvers.Measurement(ctx, prob, precond=False)
     dm = HolographicMeasurement(meas, noise_seed=2)
-    P = _oracle_problem(prob, meas).holographic(dm.noise_host)
+    P = _oracle_problem(prob, meas).holographic(dm.noise_host, datasets=_dataset_ranges(prob))
+    assert all(len(d) == n_datasets for d in prob.datasets)
     rng = np.random.default_rng(0)
     x = rng.random((prob.Ny, prob.Nx, prob.L))
     xd = torch.from_numpy(np.ascontiguousarray(x.transpose(2, 1, 0)).astype(np.float32)).cuda()
@@ -231,6 +246,8 @@ def test_dr_operator_and_adjoint(ctx):
         assert rel(yd[l][W], lam * (H @ f + H.conj().T @ f)) < TOL, l
         assert np.all(yd[l][np.setdiff1d(np.arange(P.No), W)] == 0)
         assert abs(dm.epsilon[l] - P.epsilon[l][0]) < 1e-5 * P.epsilon[l][0]
+        yg = dm.y[l * P.No:(l + 1) * P.No].cpu().numpy()
+        assert rel(yg[W], P.y[l][0]) < TOL, l
     z = np.zeros_like(yd)
     for l in range(prob.L):
         z[l][P.W[l][0]] = cplx(rng, len(P.W[l][0]))
@@ -240,21 +257,23 @@ def test_dr_operator_and_adjoint(ctx):
     assert abs(lhs - rhs) < 1e-5 * abs(lhs)
 
 
-def test_dr_hypersara_matches_oracle(ctx):
+@pytest.mark.parametrize('n_datasets', [1, 2])
+def test_dr_hypersara_matches_oracle(ctx, n_datasets):
     """HyperSARA on the DR form (imaging/imaging.m:619-631: gridded data,
     Lambda, closed-form l2 dual) against the oracle's HyperSARA on its own
-    DR problem, same operator norms: 1e-3 relative on the iterate, 0.1 dB."""
+    DR problem, same operator norms: 1e-3 relative on the iterate, 0.1 dB;
+    with two data sets per channel summed into the DR data."""
     from fhs import solvers, synth
     from fhs.holographic import HolographicMeasurement
     from oracle.solvers import hypersara as hs_ref, snr
-    prob = synth.make_problem(32, 32, 4, 1500, 2, seed=3, n_sources=20)
+    prob = synth.make_problem(32, 32, 4, 1500 // n_datasets, 2, seed=3, n_sources=20, n_datasets=n_datasets)
     meas = solvers.Measurement(ctx, prob, precond=False)
     dm = HolographicMeasurement(meas, noise_seed=1)
     son = dm.op_norm()
     K = 40
     x_gpu, info = solvers.hyper_sara(dm, {'squared_operator_norm': son, 'nu2': son}, X0=prob.X0,
                                      max_total_iter=K)
-    P = _oracle_problem(prob, meas).holographic(dm.noise_host)
+    P = _oracle_problem(prob, meas).holographic(dm.noise_host, datasets=_dataset_ranges(prob))
     sig, sig_bar = solvers.noise_levels_hs(32, 32, 4, dm.noise_std, son)
     pr = dict(solvers.DEFAULT_PARAM)
     pr.update({'nu2': son, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar, 'alpha': 1.0})

@@ -309,6 +309,35 @@ def test_holographic_operator_dense(kind):
     assert np.allclose(lam * (H @ f + H.conj().T @ f), lam * (GhG[np.ix_(W, W)] @ f))
 
 
+def test_holographic_operator_sets_sum_data_sets():
+    """Several data sets of one channel (the reference's per-set loop):
+    against a dense brute force of the summed Grams, and equal to one set of
+    the stacked rows (a sum of Grams is the Gram of the stack)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    ops = [onu.NufftOracle(rng.normal(0, 0.6, m), rng.normal(0, 0.6, m), 8, 8) for m in (30, 45, 12)]
+    Gs = [op.G.tocsr() for op in ops]
+    ys = [rng.normal(size=G.shape[0]) + 1j * rng.normal(size=G.shape[0]) for G in Gs]
+    ns = [(rng.normal(size=G.shape[0]) + 1j * rng.normal(size=G.shape[0])) / np.sqrt(2) for G in Gs]
+    H, W, lam, yg, l2, sig = odf.holographic_operator_sets(Gs, ys, ns)
+    GhG = sum(G.toarray().conj().T @ G.toarray() for G in Gs)
+    Gty = sum(G.toarray().conj().T @ y for G, y in zip(Gs, ys))
+    assert np.array_equal(W, np.flatnonzero(np.real(np.diag(GhG)) > 0))
+    Hd = H.toarray()
+    assert np.allclose(Hd + Hd.conj().T, GhG[np.ix_(W, W)])
+    assert np.allclose(lam ** 2 * np.real(np.diag(GhG))[W], 1.0)
+    assert np.allclose(yg, lam * Gty[W])
+    # one set from each data set alone is not the sum
+    H1, W1 = odf.holographic_operator_sets(Gs[:1], ys[:1])[:2]
+    assert len(W1) < len(W) or not np.allclose(H1.toarray(), Hd[np.ix_(np.searchsorted(W, W1), np.searchsorted(W, W1))])
+    # stacked rows as one set: the same DR data
+    Hs, Ws, lams, ygs, l2s, sigs = odf.holographic_operator(sp.vstack(Gs).tocsr(), np.concatenate(ys),
+                                                            np.concatenate(ns))
+    assert np.array_equal(Ws, W)
+    assert np.allclose(Hs.toarray(), Hd) and np.allclose(lams, lam) and np.allclose(ygs, yg)
+    assert np.isclose(l2s, l2) and np.isclose(sigs, sig)
+
+
 def test_dirty_image_of_point_source_is_normalised_psf():
     rng = np.random.default_rng(9)
     Ny = Nx = 16
