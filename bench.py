@@ -1,28 +1,29 @@
-"""Benchmark: HyperSARA PDFB iterations on one MI355X.
+"""Benchmark: Faceted HyperSARA PDFB iterations on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): HyperSARA,
-L = 16 channels at nu_l = nu_0 (1 + 0.05 l), 512 x 512 image, base coverage
-of 200k Gaussian uv points scaled by nu_l/nu_max with 10-30 % random
-per-channel flagging, 4 data blocks per channel, 7x7 min-max NUFFT with 2x
-oversampling, db1-db8 + Dirac at 4 levels, nuclear + weighted l21 priors,
-uniform-weight preconditioning with the ellipsoid projection, input SNR 40 dB,
-FP32 operators with FP64 norms.  Synthetic seeded data.
+Default workload (--config fhs2048x64): Faceted HyperSARA on a 2048 x 2048 x
+64 cube as C = 4 spectral sub-cubes of 16 channels, all resident on one GPU;
+6 x 6 nuclear-norm facets with 50 % overlap, 1M visibilities per channel
+(Gaussian coverage scaled by nu_l / nu_max), 4 blocks per channel, the
+w-term (w uniform in [0, 200] wavelengths, w-projected separable rows of
+15 x 15 taps), uniform-weight preconditioning, db1-db8 + Dirac at 4 levels,
+input SNR 40 dB.  Synthetic seeded data (fhs/cube.py).  A step is one PDFB
+iteration of every sub-cube (src/fhs/facetHyperSARA.m:701-889).
 
-A step is one PDFB iteration (primal update, low-rank, sparsity and data
-dual steps over all 16 channels, as in src/hs/hyperSARA.m:565-617) with its
-stopping-rule scalars read back.  Prints one JSON line.
-
-Besides iterations/s the line carries: the dominant kernel against the HBM
+The JSON line carries iterations/s of the chosen form (the PDFB engine: one
+hipGraph replay per iteration, priors on side streams, the mixed-precision
+Gram of --gram on the matrix cores), the dominant kernel against the HBM
 roofline (HIP-event timing on the engine stream, algorithmic bytes), and
-``solve``: the wall-clock and iterations to reach the reference SNR minus
-0.1 ... 3 dB for three GPU forms -- plain (the drop-in form: the
-reference-named entry points called one after the other every iteration,
-zero start), batched (the PDFB engine, one hipGraph replay per iteration)
-and warm_start (the engine from the learned start) -- with each form's
-speed-up over plain.  The reference SNR is the FP64 oracle's converged SNR
-on the same data (tests/golden/hs512x16_reference.npz) when the fixture
-exists for the workload.  The CPU baseline is the FP64 oracle on the host
-cores (10 iterations of the same problem).
+``solve``: per sub-cube the wall-clock and iterations to reach the
+reference SNR minus 0.1 ... 3 dB for the plain form (the reference-named
+drop-in entry points called one after the other every iteration, FP64
+Gram, zero start) and the chosen form, with the errors of the mixed
+Gram.  The CPU baseline is a bounded sample of one cube iteration by the
+FP64 oracle on the host.
+
+--config hs512 (HyperSARA 512 x 512 x 16, 200k base visibilities, reference
+SNR from the FP64 oracle's converged solve, tests/golden/hs512x16_reference.npz)
+also times the batched FP64-Gram engine and the learned warm start;
+hs512dr, fhs1024, fhs2048 and sara256 measure the other workloads.
 """
 import argparse
 import json
@@ -63,8 +64,8 @@ def parse():
                     help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
                          'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--gram', choices=('fp64', 'bf16x3', 'bf16x2'), default='fp64',
-                    help='nuclear-norm Gram precision of the throughput engine')
+    ap.add_argument('--gram', choices=('fp64', 'bf16x3', 'bf16x2'), default='bf16x3',
+                    help='nuclear-norm Gram precision of the chosen form (throughput engine and chosen solves)')
     ap.add_argument('--no-autotune', action='store_true',
                     help='keep the default launch shapes (the tuned ones give bit-identical results)')
     ap.add_argument('--w-max', type=float, default=None,
@@ -227,6 +228,31 @@ def sara_against_oracle(args, solvers, meas, prob, son, nu2, n_iter=300):
     return solve, cpu
 
 
+def gram_precision_errors(ctx, meas, facets, x, mode):
+    """Error of the mixed-precision Gram against the FP64 Gram (itself within
+    1e-6 of the FP64 oracle in the GPU tests) on a solve's cube x: relative
+    2-norm errors of the facets' singular values and of one low-rank dual
+    step from v0 = 0 (threshold = median singular value, unit weights)."""
+    import numpy as np
+    from fhs import LowRankPrior
+    out = {}
+    res = {}
+    for m in ('fp64', mode):
+        R = LowRankPrior(ctx, meas.Ny, meas.Nx, meas.L, facets=facets)
+        R.set_gram_precision(m)
+        sv = R.singular_values(x)
+        beta = float(np.median(res['fp64'][0])) if m != 'fp64' else float(np.median(sv))
+        v0, g0 = R.update_dual(R.zeros_v0(), x, np.ones((R.Q, R.L)), beta, 1.0)
+        res[m] = (sv, v0.double(), g0.double())
+        del R
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    out['singular_values_rel_err'] = rel(res[mode][0], res['fp64'][0])
+    out['lowrank_dual_step_rel_err'] = float((res[mode][1] - res['fp64'][1]).norm() / res['fp64'][1].norm())
+    out['lowrank_g0_rel_err'] = float((res[mode][2] - res['fp64'][2]).norm() / res['fp64'][2].norm())
+    out['reference'] = 'the FP64-Gram GPU path on the same cube (tested within 1e-6 of the FP64 oracle)'
+    return out
+
+
 def cpu_sample_cube(cfg, prob, meas, facets, threads):
     """The FP64 oracle's cost of one PDFB iteration of the whole cube,
     from a bounded sample: one channel's data step at full size (A, At and
@@ -358,41 +384,47 @@ def cube_main(args):
         fx = np.load(fx_path)
         tols = (0.1, 0.5, 1.0, 2.0, 3.0)
         per = []
+        mixed_err = None
         for c, (prob, meas, facets, param) in enumerate(subs):
             ref = float(fx[f'snr_{c}'][-1])
             prm = dict(param, track_snr=True)
             row = {'ref_snr_db': round(ref, 4), 'reference_iterations': int(fx[f'iterations_{c}']),
                    'reference_reweights': [int(t) for t in fx[f'reweights_{c}']]}
-            for form in ('dropin', 'engine'):
-                _, info = solvers._solve('fhs', meas, prm, X0=prob.X0, facets=facets, form=form,
-                                         stop_snr=ref - min(tols), time_limit=args.solve_seconds)
+            for name, kw in (('plain', dict(form='dropin')), ('chosen', dict(form='engine', gram=args.gram))):
+                eng_s, info = solvers._solve('fhs', meas, prm, X0=prob.X0, facets=facets, stop_snr=ref - min(tols),
+                                             time_limit=args.solve_seconds, **kw)
                 reach = {}
                 for d in tols:
                     k = warmstart.iterations_to(info['snr'], ref - d)
                     reach[f'ref-{d}dB'] = None if k is None else {'iterations': k, 's': round(info['time'][k - 1], 4)}
-                row['plain' if form == 'dropin' else 'batched'] = reach
+                row[name] = reach
+                if name == 'chosen' and c == 0 and args.gram != 'fp64':
+                    mixed_err = gram_precision_errors(ctx, meas, facets, eng_s.x, args.gram)
+                del eng_s
             per.append(row)
             log(f'sub-cube {c}: timed solves done')
         total = {}
-        for f in ('plain', 'batched'):
+        for f in ('plain', 'chosen'):
             total[f] = {}
             for d in tols:
                 key = f'ref-{d}dB'
                 vals = [r[f][key] for r in per]
                 total[f][key] = None if any(v is None for v in vals) else round(sum(v['s'] for v in vals), 4)
-        speed = {key: (None if total['plain'][key] is None or total['batched'][key] is None else
-                       round(total['plain'][key] / total['batched'][key], 3)) for key in total['plain']}
+        speed = {key: (None if total['plain'][key] is None or total['chosen'][key] is None else
+                       round(total['plain'][key] / total['chosen'][key], 3)) for key in total['plain']}
         solve = {'ref_definition': ('per sub-cube, the converged SNR of the plain GPU solve with the reference '
                                     'stopping rules and 3 reweighting steps (tests/golden/fhs2048x64_reference.npz; '
                                     'no FP64 solve of this size is feasible -- the solver loop is checked against '
                                     'the FP64 oracle at small sizes); SNR = 20 log10(||X0||/||X0 - X||)'),
                  'timing': 'the sub-cubes are independent solves run one after the other: the cube reaches a '
                            'tolerance when every sub-cube has, at the sum of their times',
-                 'plain_form': 'drop-in entry points per iteration, eager, zero start',
-                 'batched_form': 'PDFB engine (one hipGraph replay per iteration, priors on side streams, stopping '
-                                 'rule in the library), zero start',
-                 'warm_start': 'not run at this size (the shipped network is trained on 512x512x16 cubes)',
-                 'subcubes': per, 'total_s': total, 'speedup_batched_vs_plain': speed}
+                 'plain_form': 'drop-in entry points per iteration, eager, FP64 Gram, zero start',
+                 'chosen_form': (f'PDFB engine (one hipGraph replay per iteration, priors on side streams, stopping '
+                                 f'rule in the library), {args.gram} Gram on the matrix cores, zero start'),
+                 'warm_start': ('not part of the chosen form: the learned start does not shorten the time to '
+                                'ref - 0.1 dB at 512x512x16, and the shipped network is trained on 512x512x16 cubes'),
+                 'mixed_precision_errors': mixed_err,
+                 'subcubes': per, 'total_s': total, 'speedup_chosen_vs_plain': speed}
     # ---- CPU baseline: bounded oracle sample --------------------------------
     cpu = None
     if not args.no_cpu:
@@ -413,7 +445,7 @@ def cube_main(args):
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f32 (f64 norms, f64 Gram/eig)',
+        'dtype': f'f32 (f64 norms, {args.gram} Gram, f64 eig)',
         'data': 'synthetic',
         'config': {'workload': (f'Faceted HyperSARA {cfg["N"]}x{cfg["N"]}x{cfg["channels"]} as C = {C} spectral '
                                 f'sub-cubes of {cfg["channels"] // C} channels solved on one GPU (one step = one PDFB '
@@ -619,8 +651,23 @@ def main():
                              'iterations': full['iterations'], 'wall_s': round(full['wall'], 3),
                              'converged': full['converged'], 'reweights': full['reweights'],
                              'final_snr_db': round(full['snr'][-1], 4), 'reach': b_reach, 'reach_s_each': b_all}}
-        wpath = warmstart.default_weights_path()
         forms = ['batched']
+        if kind != 'sara':
+            # chosen: the engine with the mixed-precision Gram (matrix cores)
+            eng_c, chosen = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets,
+                                           time_limit=args.solve_seconds, gram=args.gram)
+            mixed_err = gram_precision_errors(ctx, meas, facets, eng_c.x, args.gram) if args.gram != 'fp64' else None
+            del eng_c
+            log(f'chosen-form solve: {chosen["iterations"]} iterations, SNR {chosen["snr"][-1]:.4f} dB')
+            c_reach, c_all = reach(timed(gram=args.gram))
+            solve['chosen'] = {'form': f'PDFB engine as batched, with the {args.gram} Gram on the matrix cores',
+                               'iterations': chosen['iterations'], 'wall_s': round(chosen['wall'], 3),
+                               'converged': chosen['converged'], 'reweights': chosen['reweights'],
+                               'final_snr_db': round(chosen['snr'][-1], 4),
+                               'final_minus_ref_db': round(chosen['snr'][-1] - ref, 4),
+                               'mixed_precision_errors': mixed_err, 'reach': c_reach, 'reach_s_each': c_all}
+            forms.append('chosen')
+        wpath = warmstart.default_weights_path()
         if os.path.exists(wpath) and not dr:
             ws = warmstart.WarmStart.load(ctx, wpath)
             ws.predict(warmstart.dirty_cube(meas))      # one untimed call: workspaces and code objects
@@ -698,7 +745,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f32 (f64 norms, f64 Gram/eig)',
+        'dtype': f'f32 (f64 norms, {args.gram if kind != "sara" else "no"} Gram, f64 eig)',
         'data': 'synthetic',
         'config': {'workload': (f'SARA {args.size}x{args.size}, {args.vis} visibilities, 2 blocks'
                                 if args.config == 'sara256' else
