@@ -18,7 +18,11 @@ files, checkpoints and FITS cubes.
   fields of ``src/fhs/facetHyperSARA.m:1165-1205`` (``src/hs/hyperSARA.m``
   the same names): ``param, res, g, xsol, epsilon, v2, proj, t_block,
   norm_res, v0, v1, weights0, weights1, end_iter, t_facet, t_data,
-  rel_val`` (cells as object arrays).
+  rel_val`` (cells as object arrays, nested cells kept as nested object
+  arrays).  The exchange works one way as it stands: the reference reads the
+  v5 files written here, but it writes its own checkpoints through
+  ``matfile(..., 'Writable', true)`` (``:1165``), which makes MAT v7.3
+  (HDF5) files; those must be re-saved with ``-v7`` before they load here.
 - ``fitswrite`` / ``fitsread``: the primary-HDU FITS files of the model and
   residual cubes (``..._FINAL_WB_MODEL.fits``, ``..._FINAL_WB_RESIDUAL.fits``,
   ``src/fhs/facetHyperSARA.m:1218-1219``) and of initial images

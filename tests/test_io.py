@@ -131,3 +131,45 @@ def test_checkpoint_fields(tmp_path):
     assert np.allclose(c['v2'][0], v2[0]) and np.allclose(c['v2'][1], v2[1])
     assert np.allclose(c['epsilon'][0], eps[0]) and np.allclose(c['v0'][0], v0[0])
     assert int(c['param']['pdfb_max_iter'][0, 0].squeeze()) == 2000
+
+
+def test_reference_checkpoint_layout(tmp_path):
+    """A checkpoint with the reference's exact field layout
+    (src/fhs/facetHyperSARA.m:1165-1219: per data worker k, epsilon / v2 /
+    proj / t_block / norm_res hold a cell per channel of a cell per block;
+    param carries the init_* restart fields), as MATLAB writes it after a
+    re-save with -v7 (the solver's own matfile(..., 'Writable', true) writes
+    v7.3, which is refused here with a message)."""
+    import scipy.io as sio
+    rng = np.random.default_rng(4)
+    Ny, Nx, L, K, Q = 6, 5, 2, 2, 2
+
+    def cell(items):
+        c = np.empty((len(items), 1), dtype=object)
+        for i, it in enumerate(items):
+            c[i, 0] = it
+        return c
+    blocks = [[3, 2], [4]]                            # per channel of a worker: block sizes
+    v2 = cell([cell([cell([rng.normal(size=(m, 1)) + 1j * rng.normal(size=(m, 1)) for m in blocks[c]])
+                     for c in range(len(blocks))]) for _ in range(K)])
+    eps = cell([cell([cell([np.array([[1.5]]) for _ in blocks[c]]) for c in range(len(blocks))]) for _ in range(K)])
+    m = {'param': {'pdfb_max_iter': 2000.0, 'reweighting_alpha': 1.0, 'init_reweight_step_count': 2.0,
+                   'init_reweight_last_iter_step': 640.0, 'init_t_start': 641.0},
+         'res': rng.normal(size=(Ny, Nx, L)), 'g': rng.normal(size=(Ny, Nx, L)), 'xsol': rng.random((Ny, Nx, L)),
+         'epsilon': eps, 'v2': v2, 'proj': v2, 't_block': eps, 'norm_res': eps,
+         'v0': cell([rng.normal(size=(12, L)) for _ in range(Q)]), 'v1': cell([rng.normal(size=(50, L))
+                                                                                for _ in range(Q)]),
+         'weights0': cell([np.ones((L, 1)) for _ in range(Q)]), 'weights1': cell([np.ones((50, 1))
+                                                                                 for _ in range(Q)]),
+         'end_iter': np.arange(641.0)[None, :], 't_facet': np.zeros((1, 641)), 't_data': np.zeros((1, 641))}
+    p = tmp_path / 'fhs_cube_rw2.mat'
+    sio.savemat(p, m, format='5')
+    c = fio.load_checkpoint(p)
+    assert set(c) == set(fio.CHECKPOINT_FIELDS) - {'rel_val'}
+    assert np.array_equal(c['xsol'], m['xsol'])
+    assert len(c['v2']) == K and len(c['v0']) == Q
+    # worker k, channel 1, block 0: the nested cells survive
+    w = c['v2'][1]
+    assert np.allclose(np.asarray(w.ravel()[1].ravel()[0]).ravel(), v2[1, 0][1, 0][0, 0].ravel())
+    assert int(np.asarray(c['param']['init_t_start'][0, 0]).squeeze()) == 641
+    assert c['end_iter'].size == 641
