@@ -15,4 +15,4 @@ from .operators import (  # noqa: F401
 )
 from .data_fidelity import DataOperator, update_dual_data_fidelity  # noqa: F401,E402
 from .lowrank import LowRankPrior, hs_update_dual_lowrankness_serial, nuclear_norm  # noqa: F401,E402
-from .sara import SaraDictionary, compute_sara_prior, hs_update_dual_sparsity_distributed  # noqa: F401,E402
+from .sara import SaraDictionary, SaraFacets, compute_sara_prior, hs_update_dual_sparsity_distributed  # noqa: F401,E402,E501

@@ -83,6 +83,14 @@ PROTOTYPES = {
     'fhs_sara_info': [c_void_p, c_i64_p],
     'fhs_sara_analysis': [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int],
     'fhs_sara_synthesis': [c_void_p, c_void_p, c_void_p, c_int, C.c_float, c_void_p],
+    'fhs_sara_facets_create': [c_void_p, c_int, c_int, c_int, c_int, c_int_p, c_double_p, c_int, c_int, c_int,
+                               c_int_p, c_int_p, C.POINTER(c_void_p)],
+    'fhs_sara_facets_destroy': [c_void_p],
+    'fhs_sara_facets_info': [c_void_p, c_int, c_int_p, c_i64_p, c_int_p],
+    'fhs_sara_facet_analysis': [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p],
+    'fhs_sara_facet_synthesis': [c_void_p, c_void_p, c_int, c_void_p, c_int, C.c_float, c_void_p],
+    'fhs_update_dual_sparsity_facet': [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_double,
+                                       c_void_p],
     'fhs_update_dual_sparsity': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
                                  C.c_float, c_void_p],
     'fhs_compute_sara_prior': [c_void_p, c_void_p, c_void_p, c_int, c_double_p],

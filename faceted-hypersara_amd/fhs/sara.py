@@ -95,6 +95,121 @@ class SaraDictionary:
             self.h = None
 
 
+class SaraFacets:
+    """Faceted SARA transform (``fhs_sara_facets``): the role of the
+    reference's ``sdwt2_setup`` / ``sdwt2_sara_faceting`` /
+    ``isdwt2_sara_faceting`` (``src/fhs/facetHyperSARA.m:325-326``,
+    ``src/fhs/fhs_update_dual_sparsity.m:71-93``).  ``ybounds`` /
+    ``xbounds``: facet boundaries from 0 to the image size; facet
+    q = qx * Qy + qy.  Every facet reads an extended image (``ext(q)``: its
+    pixels plus a top/left border, zero outside the image) and owns an exact
+    share of the whole-image coefficients."""
+
+    def __init__(self, ctx, Ny, Nx, ybounds, xbounds, bases=DEFAULT_BASES, nlevel=DEFAULT_NLEVEL):
+        self.ctx, self.Ny, self.Nx, self.nlevel = ctx, Ny, Nx, nlevel
+        bases = list(bases)
+        dirac = 'self' in bases
+        if dirac and bases[-1] != 'self':
+            raise ValueError("SARA: the Dirac basis 'self' must come last")
+        db = [b for b in bases if b != 'self']
+        filt = [daubechies_scaling_filter(int(b[2:])) for b in db]
+        lens = np.asarray([len(f) for f in filt], dtype=np.int32)
+        lo = np.ascontiguousarray(np.concatenate(filt) if filt else np.zeros(1))
+        yb = np.ascontiguousarray(ybounds, dtype=np.int32)
+        xb = np.ascontiguousarray(xbounds, dtype=np.int32)
+        self.Qy, self.Qx = len(yb) - 1, len(xb) - 1
+        self.Q = self.Qy * self.Qx
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))   # noqa: E731
+        h = C.c_void_p()
+        ctx.check(lib().fhs_sara_facets_create(ctx.h, Ny, Nx, nlevel, len(db), ip(lens),
+                                               lo.ctypes.data_as(C.POINTER(C.c_double)), int(dirac), self.Qy,
+                                               self.Qx, ip(yb), ip(xb), C.byref(h)), 'sara_facets_create')
+        self.h = h
+        nrows = len(db) * (nlevel + 1) + int(dirac)
+        self._ext, self._n, self.ncoefs = [], [], []
+        for q in range(self.Q):
+            e = np.zeros(4, dtype=np.int32)
+            n = C.c_int64()
+            rows = np.zeros(2 * nrows, dtype=np.int32)
+            lib().fhs_sara_facets_info(h, q, ip(e), C.byref(n), ip(rows))
+            self._ext.append(tuple(int(v) for v in e))
+            self._n.append(n.value)
+            self.ncoefs.append(rows.reshape(nrows, 2))
+
+    def ext(self, q):
+        """(row origin, col origin, rows, cols) of facet q's extended image."""
+        return self._ext[q]
+
+    def n_coefs(self, q):
+        return self._n[q]
+
+    def extended(self, q, x):
+        """Facet q's extended image cut from a whole cube x (L, Nx, Ny), zero
+        outside the image: (L, enx, eny)."""
+        ey0, ex0, eny, enx = self._ext[q]
+        xx = x.reshape(-1, self.Nx, self.Ny)
+        out = torch.zeros((xx.shape[0], enx, eny), dtype=torch.float32, device=xx.device)
+        y0, x0 = max(ey0, 0), max(ex0, 0)
+        out[:, x0 - ex0:, y0 - ey0:] = xx[:, x0:ex0 + enx, y0:ey0 + eny]
+        return out
+
+    def add_extended(self, q, g, out):
+        """out (L, Nx, Ny) += facet q's extended-region image g, cropped."""
+        ey0, ex0, eny, enx = self._ext[q]
+        y0, x0 = max(ey0, 0), max(ex0, 0)
+        out[:, x0:ex0 + enx, y0:ey0 + eny] += g[:, x0 - ex0:, y0 - ey0:]
+        return out
+
+    def analysis(self, q, x_ext):
+        """z = sdwt2_sara_faceting(x_ext): (L, n_coefs(q))."""
+        _require(x_ext, torch.float32, self.ctx.device, 'facet analysis: x_ext')
+        ey0, ex0, eny, enx = self._ext[q]
+        L = x_ext.numel() // (eny * enx)
+        if L * eny * enx != x_ext.numel():
+            raise ValueError('facet analysis: x_ext must be (L, enx, eny)')
+        z = torch.empty((L, self._n[q]), dtype=torch.float32, device=self.ctx.device)
+        self.ctx.check(lib().fhs_sara_facet_analysis(self.ctx.h, self.h, q, _ptr(x_ext), L, _ptr(z)),
+                       'sara_facet_analysis')
+        return z
+
+    def synthesis(self, q, z, scale=1.0):
+        """scale * isdwt2_sara_faceting(z): (L, enx, eny)."""
+        _require(z, torch.float32, self.ctx.device, 'facet synthesis: z')
+        ey0, ex0, eny, enx = self._ext[q]
+        L = z.numel() // self._n[q]
+        g = torch.empty((L, enx, eny), dtype=torch.float32, device=self.ctx.device)
+        self.ctx.check(lib().fhs_sara_facet_synthesis(self.ctx.h, self.h, q, _ptr(z), L, float(scale), _ptr(g)),
+                       'sara_facet_synthesis')
+        return g
+
+    def update_dual(self, q, v1, x_ext, weights1, beta1):
+        """[v1, g] = fhs_update_dual_sparsity(v1, x_facet, weights1, beta1, ...)
+        for facet q (``src/fhs/fhs_update_dual_sparsity.m:65-93``): v1
+        (L, n_coefs(q)) updated in place; returns (v1, g_ext)."""
+        _require(v1, torch.float32, self.ctx.device, 'facet dual: v1')
+        _require(x_ext, torch.float32, self.ctx.device, 'facet dual: x_ext')
+        ey0, ex0, eny, enx = self._ext[q]
+        L = x_ext.numel() // (eny * enx)
+        if v1.numel() != L * self._n[q]:
+            raise ValueError('facet dual: v1 must be (L, n_coefs)')
+        if weights1 is not None:
+            _require(weights1, torch.float32, self.ctx.device, 'facet dual: weights1')
+        g = torch.empty((L, enx, eny), dtype=torch.float32, device=self.ctx.device)
+        self.ctx.check(lib().fhs_update_dual_sparsity_facet(self.ctx.h, self.h, q, _ptr(v1), _ptr(x_ext), L,
+                                                            _ptr(weights1), float(beta1), _ptr(g)),
+                       'update_dual_sparsity_facet')
+        return v1, g
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h is not None:
+            try:
+                lib().fhs_sara_facets_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
 def hs_update_dual_sparsity_distributed(v1, Psit, Psi, xhat, weights1, beta1, sigma1):
     """``[v1, g1] = hs_update_dual_sparsity_distributed(v1, Psit, Psi, xhat,
     weights1, beta1, sigma1)`` — ``src/hs/hs_update_dual_sparsity_distributed.m:4-17``.

@@ -40,6 +40,7 @@ typedef struct fhs_nufft fhs_nufft;   /* oversampled scaled FFT plan (A / At) */
 typedef struct fhs_sara fhs_sara;     /* SARA dictionary plan (Psit / Psi) */
 typedef struct fhs_data fhs_data;     /* data-fidelity state of L channels */
 typedef struct fhs_lowrank fhs_lowrank; /* facet layout of the nuclear-norm prior */
+typedef struct fhs_sara_facets fhs_sara_facets; /* faceted SARA transform (per-facet Psit / Psi) */
 
 /* ------------------------------------------------------------ context -- */
 int fhs_ctx_create(int device, fhs_ctx** out);
@@ -324,6 +325,40 @@ int fhs_sara_update_weights(fhs_ctx* ctx, const fhs_sara* S, const float* x, int
 /* f = compute_sara_prior(x, Psit, n_wavelet_coefficients)
  * src/compute_sara_prior.m:33-41 (host double out). */
 int fhs_compute_sara_prior(fhs_ctx* ctx, const fhs_sara* S, const float* x, int L, double* out);
+
+/* Faceted SARA transform over a grid of non-overlapping facets (the
+ * reference's faceted wavelet submodule, absent from its tree: sdwt2_setup
+ * src/fhs/facetHyperSARA.m:325-326, sdwt2_sara_faceting /
+ * isdwt2_sara_faceting src/fhs/fhs_update_dual_sparsity.m:71-93).  Facet
+ * q = qx*Qy + qy owns pixels [ybounds[qy], ybounds[qy+1]) x [xbounds[qx],
+ * xbounds[qx+1]); dictionary arguments as fhs_sara_create_host.  Each facet
+ * reads an extended image (its pixels plus a top/left border from its
+ * neighbours, zero outside the image) and owns an exact share of the
+ * whole-image coefficients; the sum of the facets' syntheses is the
+ * whole-image synthesis. */
+int fhs_sara_facets_create(fhs_ctx* ctx, int Ny, int Nx, int nlevel, int nbases, const int* filter_len,
+                           const double* lo_r, int with_dirac, int Qy, int Qx, const int* ybounds,
+                           const int* xbounds, fhs_sara_facets** out);
+int fhs_sara_facets_destroy(fhs_sara_facets* F);
+/* ext: int[4] = origin (row, col; may be negative) and size (rows, cols) of
+ * facet q's extended image; n_coefs: its coefficients per channel;
+ * ncoefs_rows (or NULL): int[2 * (nbases (nlevel + 1) + with_dirac)], per
+ * basis the (rows, cols) of its levels 1..nlevel and of the approximation,
+ * the Dirac part last (the rows of the reference's Ncoefs{q}). */
+int fhs_sara_facets_info(const fhs_sara_facets* F, int q, int* ext, int64_t* n_coefs, int* ncoefs_rows);
+/* z = sdwt2_sara_faceting(x_ext): x_ext float[L][enx][eny], z float[L][n_coefs] */
+int fhs_sara_facet_analysis(fhs_ctx* ctx, const fhs_sara_facets* F, int q, const float* x_ext, int L, float* z);
+/* g_ext = scale * isdwt2_sara_faceting(z): float[L][enx][eny], facet q's
+ * contribution over its extended image */
+int fhs_sara_facet_synthesis(fhs_ctx* ctx, const fhs_sara_facets* F, int q, const float* z, int L, float scale,
+                             float* g_ext);
+/* [v1, g] = fhs_update_dual_sparsity(v1, x_facet, weights1, beta1, ...)
+ * src/fhs/fhs_update_dual_sparsity.m:65-93 for facet q: w = v1 + Psit_q
+ * x_ext; v1 <- w - max(||w|| - beta1 weights1, 0)/(||w|| + eps) w (norms
+ * over the L channels); g_ext = Psi_q v1.  v1 float[L][n_coefs], weights1
+ * float[n_coefs] or NULL (= 1). */
+int fhs_update_dual_sparsity_facet(fhs_ctx* ctx, const fhs_sara_facets* F, int q, float* v1, const float* x_ext,
+                                   int L, const float* weights1, double beta1, float* g_ext);
 
 /* -------------------------------------------------- low-rank prior -- */
 /* Facets q = qx*Qy + qy (src/fhs/facetHyperSARA.m:310-321 order) with origin

@@ -196,3 +196,126 @@ class SaraDictionary:
             else:
                 out += waverec2_adjoint(ci, b, self.nlevel, self.Ny, self.Nx)
         return out / np.sqrt(self.P)
+
+
+# ---- faceted SARA transform -------------------------------------------------
+# The reference splits the analysis and synthesis over non-overlapping facets
+# (sdwt2_setup / sdwt2_sara_faceting / isdwt2_sara_faceting of its faceted
+# wavelet submodule, absent from the reference tree; call sites
+# src/fhs/facetHyperSARA.m:325-326, src/fhs/fhs_update_dual_sparsity.m:71-93,
+# src/fhs/fhs_initialize_dual_and_weights.m:80-103).  The facet transforms
+# are exact pieces of the global one: every global coefficient belongs to
+# one facet, and a facet computes its coefficients from its own pixels plus a
+# top/left extension that the neighbouring facets provide.  The ownership
+# rule below is this project's (the submodule's own layout is not in the
+# tree): at level j a facet owns the coefficients k whose last input sample
+# 2k + 1 lies in the range it owns at level j - 1 (the level-0 ranges being
+# its pixels), the right-border coefficients of the zpd extension going to
+# the last facet.  So the level-j boundaries are t = ceil((s - 1) / 2) of
+# the level-(j - 1) boundaries s.
+
+def facet_level_bounds(N, starts, lf, nlevel):
+    """Boundaries of every level's owned ranges along one axis: bounds[j][i]
+    .. bounds[j][i + 1] is facet i's range at level j (level 0 = pixels);
+    ``starts``: ascending facet origins, starts[0] = 0."""
+    bounds = [list(starts) + [N]]
+    n = N
+    for _ in range(nlevel):
+        n = n_out(n, lf)
+        prev = bounds[-1]
+        bounds.append([0] + [s // 2 for s in prev[1:-1]] + [n])   # ceil((s - 1) / 2) = floor(s / 2)
+    return bounds
+
+
+def facet_extension_start(t_J, lf, nlevel):
+    """First input pixel the owned coefficients of a facet need, from its
+    first owned level-J coefficient: m_(j-1) = 2 m_j + 2 - lf; rounded down
+    to a multiple of 2^nlevel so that the extended image decimates in phase
+    with the whole image."""
+    m = t_J
+    for _ in range(nlevel):
+        m = 2 * m + 2 - lf
+    step = 1 << nlevel
+    return (m // step) * step
+
+
+class SaraFacets:
+    """Faceted Psit / Psi: facet (qy, qx) = q = qx * Qy + qy owns pixels
+    [ys[qy], ys[qy+1]) x [xs[qx], xs[qx+1]) and reads the extended image
+    [ey0, ys[qy+1]) x [ex0, xs[qx+1]) (zero where the origin is negative).
+
+    Facet coefficient vector: per basis the owned tiles of A_J, then H_j,
+    V_j, D_j for j = J .. 1 (each column-major), the Dirac part last (the
+    owned pixels); scaled by 1/sqrt(P) like ``SaraDictionary``."""
+
+    def __init__(self, Ny, Nx, ys, xs, bases=DEFAULT_BASES, nlevel=DEFAULT_NLEVEL):
+        self.Ny, self.Nx, self.nlevel = Ny, Nx, nlevel
+        self.bases = tuple(bases)
+        self.ys, self.xs = list(ys), list(xs)
+        self.Qy, self.Qx = len(ys) - 1, len(xs) - 1
+        self.glob = SaraDictionary(Ny, Nx, bases, nlevel)
+        lfs = [len(filters(b)[0]) for b in self.bases if b != 'self']
+        by = {lf: facet_level_bounds(Ny, self.ys[:-1], lf, nlevel) for lf in set(lfs)}
+        bx = {lf: facet_level_bounds(Nx, self.xs[:-1], lf, nlevel) for lf in set(lfs)}
+        self.facets = []
+        for qx in range(self.Qx):
+            for qy in range(self.Qy):
+                ey0 = min([facet_extension_start(by[lf][nlevel][qy], lf, nlevel) for lf in lfs] + [self.ys[qy]])
+                ex0 = min([facet_extension_start(bx[lf][nlevel][qx], lf, nlevel) for lf in lfs] + [self.xs[qx]])
+                eny, enx = self.ys[qy + 1] - ey0, self.xs[qx + 1] - ex0
+                sub = SaraDictionary(eny, enx, bases, nlevel)
+                loc, glb = [], []
+                for i, b in enumerate(self.bases):
+                    if b == 'self':
+                        r = np.arange(self.ys[qy], self.ys[qy + 1])
+                        c = np.arange(self.xs[qx], self.xs[qx + 1])
+                        loc.append(sub.offsets[i] + ((r - ey0)[:, None] + eny * (c - ex0)[None, :]).ravel(order='F'))
+                        glb.append(self.glob.offsets[i] + (r[:, None] + Ny * c[None, :]).ravel(order='F'))
+                        continue
+                    lf = len(filters(b)[0])
+                    ssh = level_shapes(eny, enx, lf, nlevel)
+                    gsh = level_shapes(Ny, Nx, lf, nlevel)
+                    # subband order inside a basis: A_J, then (H, V, D)_j for j = J .. 1
+                    order = [('a', nlevel)] + [(sb, j) for j in range(nlevel, 0, -1) for sb in 'hvd']
+                    so, go = 0, 0
+                    for sb, j in order:
+                        sny, snx = ssh[j]
+                        gny, gnx = gsh[j]
+                        r = np.arange(by[lf][j][qy], by[lf][j][qy + 1])
+                        c = np.arange(bx[lf][j][qx], bx[lf][j][qx + 1])
+                        dy, dx = ey0 >> j, ex0 >> j
+                        loc.append(sub.offsets[i] + so + ((r - dy)[:, None] + sny * (c - dx)[None, :]).ravel(order='F'))
+                        glb.append(self.glob.offsets[i] + go + (r[:, None] + gny * c[None, :]).ravel(order='F'))
+                        so += sny * snx
+                        go += gny * gnx
+                loc = np.concatenate(loc).astype(np.int64)
+                glb = np.concatenate(glb).astype(np.int64)
+                self.facets.append({'ey0': ey0, 'ex0': ex0, 'eny': eny, 'enx': enx, 'sub': sub,
+                                    'local': loc, 'global': glb})
+
+    def extended(self, q, x):
+        """The extended image of facet q cut from a whole image x (zero padded)."""
+        f = self.facets[q]
+        out = np.zeros((f['eny'], f['enx']))
+        y0, x0 = max(f['ey0'], 0), max(f['ex0'], 0)
+        y1, x1 = f['ey0'] + f['eny'], f['ex0'] + f['enx']
+        out[y0 - f['ey0']:, x0 - f['ex0']:] = x[y0:y1, x0:x1]
+        return out
+
+    def analysis(self, q, x_ext):
+        """sdwt2_sara_faceting: the facet's coefficients from its extended image."""
+        f = self.facets[q]
+        return f['sub'].psit(x_ext)[f['local']]
+
+    def synthesis(self, q, z):
+        """isdwt2_sara_faceting: the facet's contribution (extended region)."""
+        f = self.facets[q]
+        c = np.zeros(f['sub'].s)
+        c[f['local']] = z
+        return f['sub'].psi(c)
+
+    def add_extended(self, q, g, out):
+        """out += the facet's extended-region image g, cropped to the image."""
+        f = self.facets[q]
+        y0, x0 = max(f['ey0'], 0), max(f['ex0'], 0)
+        out[y0:f['ey0'] + f['eny'], x0:f['ex0'] + f['enx']] += g[y0 - f['ey0']:, x0 - f['ex0']:]

@@ -171,3 +171,51 @@ def test_mixed_precision_gram(ctx, mode, tol_sv, tol_prox, Ny, Nx, Qy, Qx, L):
     assert errs['fp64'][0] < TOL and errs['fp64'][1] < TOL
     assert errs[mode][0] < tol_sv, errs[mode]
     assert errs[mode][1] < tol_prox and errs[mode][2] < tol_prox, errs[mode]
+
+
+@pytest.mark.parametrize('Ny,Nx,ys,xs,L', [(64, 48, [0, 20, 41, 64], [0, 25, 48], 3),
+                                           (128, 96, [0, 64, 128], [0, 30, 61, 96], 4),
+                                           (40, 40, [0, 40], [0, 40], 2)])
+def test_faceted_sara_transform(ctx, Ny, Nx, ys, xs, L):
+    """Per-facet SARA analysis, synthesis and dual step (the faceted wavelet
+    transform of src/fhs/fhs_update_dual_sparsity.m:71-93) against the
+    oracle's faceted transform; the facets' syntheses add up to the
+    whole-image synthesis; the facet sizes follow the reference's coefficient
+    count sz = 3 sum(p) - 2 sum(p(nlevel+1:nlevel+1:end)) - 2 p(end), p =
+    prod(Ncoefs, 2) (src/fhs/fhs_initialize_dual_and_weights.m:80-81)."""
+    import fhs
+    from oracle.wavelets import SaraFacets as FacetsRef
+    rng = np.random.default_rng(Ny + L)
+    ref = FacetsRef(Ny, Nx, ys, xs)
+    F = fhs.SaraFacets(ctx, Ny, Nx, ys, xs)
+    S = fhs.SaraDictionary(ctx, Ny, Nx)
+    x = rng.random((Ny, Nx, L)).astype(np.float32).astype(np.float64)
+    xd = cube_to_dev(x)
+    c = rng.normal(size=(S.s, L)).astype(np.float32)
+    cd = torch.from_numpy(np.ascontiguousarray(c.T)).cuda()
+    total = torch.zeros((L, Nx, Ny), dtype=torch.float32, device='cuda')
+    for q, f in enumerate(ref.facets):
+        assert F.ext(q) == (f['ey0'], f['ex0'], f['eny'], f['enx'])
+        n = F.n_coefs(q)
+        assert n == len(f['local'])
+        p = np.prod(F.ncoefs[q].astype(np.int64), axis=1)
+        sz = 3 * p.sum() - 2 * p[S.nlevel::S.nlevel + 1].sum() - 2 * p[-1]
+        assert sz == n, (sz, n)
+        xe = F.extended(q, xd)
+        z = F.analysis(q, xe).cpu().numpy()
+        zr = np.stack([ref.analysis(q, ref.extended(q, x[:, :, l])) for l in range(L)])
+        assert rel(z, zr) < TOL, q
+        # synthesis of this facet's share of a whole-image coefficient vector
+        zq = cd[:, torch.from_numpy(f['global']).cuda()].contiguous()
+        F.add_extended(q, F.synthesis(q, zq), total)
+        # dual step
+        v1 = rng.normal(size=(L, n)).astype(np.float32)
+        w1 = rng.uniform(0.5, 1.0, n).astype(np.float32)
+        beta = 0.3
+        v1d, g = F.update_dual(q, torch.from_numpy(v1).cuda(), xe, torch.from_numpy(w1).cuda(), beta)
+        vr = soft_l21((v1.astype(np.float64) + zr).T, beta * w1.astype(np.float64)).T
+        gr = np.stack([ref.synthesis(q, vr[l]) for l in range(L)])
+        assert rel(v1d.cpu().numpy(), vr) < TOL, q
+        assert rel(g.cpu().numpy().transpose(0, 2, 1), gr) < TOL, q
+    whole = S.psi(cd)
+    assert rel(total.cpu().numpy(), whole.cpu().numpy()) < TOL

@@ -409,3 +409,36 @@ def test_product_facet_geometry_matches_oracle(N, Q, f):
         assert list(np.atleast_1d(a[k])) == list(np.atleast_1d(b[k])), k
     for wa, wb in zip(a['windows'], b['windows']):
         assert np.array_equal(wa, wb)
+
+
+@pytest.mark.parametrize('Ny,Nx,ys,xs', [(64, 48, [0, 20, 41, 64], [0, 25, 48]),
+                                         (100, 90, [0, 50, 100], [0, 30, 61, 90]),
+                                         (37, 29, [0, 37], [0, 29])])
+def test_faceted_sara_is_the_global_transform(Ny, Nx, ys, xs):
+    """The faceted SARA transform (uneven facets): every global coefficient is
+    owned by exactly one facet, each facet's analysis of its extended image
+    gives those global coefficients, and the facets' syntheses overlap-added
+    give the global synthesis (the exact pieces the reference's faceted
+    wavelet transform is built to produce)."""
+    from oracle.wavelets import SaraFacets
+    rng = np.random.default_rng(Ny)
+    F = SaraFacets(Ny, Nx, ys, xs)
+    x = rng.random((Ny, Nx))
+    g = F.glob.psit(x)
+    owned = np.concatenate([f['global'] for f in F.facets])
+    assert np.array_equal(np.sort(owned), np.arange(F.glob.s))
+    for q, f in enumerate(F.facets):
+        assert np.allclose(F.analysis(q, F.extended(q, x)), g[f['global']], rtol=0, atol=1e-13)
+        assert f['ey0'] % 16 == 0 and f['ex0'] % 16 == 0
+    c = rng.random(F.glob.s)
+    out = np.zeros((Ny, Nx))
+    for q, f in enumerate(F.facets):
+        F.add_extended(q, F.synthesis(q, c[f['global']]), out)
+    assert np.allclose(out, F.glob.psi(c), rtol=0, atol=1e-12)
+    # a facet's coefficients do not depend on pixels outside its extended image
+    q = len(F.facets) - 1
+    f = F.facets[q]
+    x2 = x.copy()
+    x2[:max(f['ey0'], 0), :] = 5.0
+    x2[:, :max(f['ex0'], 0)] = 5.0
+    assert np.allclose(F.glob.psit(x2)[f['global']], g[f['global']], rtol=0, atol=1e-13)
