@@ -482,9 +482,12 @@ def main():
         a = torch.ones(n // 4, dtype=torch.float32, device='cuda')
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            b.copy_(a)             # the runtime's copy kernel
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)   # an elementwise kernel (vector loads and stores)
         torch.cuda.synchronize()
-        print(json.dumps({'copy_bytes_read': n, 'copy_bytes_written': n, 'repeats': 3}))
+        print(json.dumps({'copy_bytes_read': n, 'copy_bytes_written': n, 'repeats': 3,
+                          'kernels': ['copy', 'elementwise add']}))
         return
     if args.config == 'fhs2048x64':
         return cube_main(args)
