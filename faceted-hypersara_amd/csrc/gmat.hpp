@@ -260,7 +260,10 @@ __device__ __forceinline__ void g_adjoint_unit(const GDev& g, int u, W&& w, X&& 
 //    inner dimension runs over the tile's rows.  Each of the 4 waves owns
 //    one 16 x 16 quadrant, accumulated by v_mfma_f32_16x16x4f32 in FP32, two
 //    complex rows per instruction pair (K = 4 real terms, re/im split over
-//    the A/B lanes), over the rows whose window meets its quadrant.  A
+//    the A/B lanes), over the rows whose window meets its quadrant.  Rows
+//    are staged 32 at a time by all 256 threads (8 per row, coalesced), the
+//    next chunk loaded during this chunk's updates; 32-row chunks keep the
+//    LDS at 17 KB, so eight workgroups fit on a CU (the register limit).  A
 //    7 x 7 window fills under a fifth of such an update, which is why the
 //    narrow rows keep the gather.
 // Every output point is written once, by one thread, after a fixed-order
@@ -312,7 +315,7 @@ template <int CH>
 __host__ __device__ constexpr size_t tile_mfma_lds_bytes() {
   return 2 * (size_t)(CH * TM_STRIDE + TM_GAP) * 8 + (size_t)CH * 4 + (size_t)4 * (CH + 4) * 4;
 }
-constexpr int TILE_MCH = 64;                          // rows staged per chunk (matrix cores)
+constexpr int TILE_MCH = 32;                          // rows staged per chunk (matrix cores)
 
 __host__ __device__ inline size_t tile_lds_bytes(int Ky, int Kx) {
   return (Ky > 8 || Kx > 8) ? tile_mfma_lds_bytes<TILE_MCH>() : tile_lds_bytes_gather(Ky, Kx);
@@ -356,26 +359,57 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
     C[k] = make_float2(0.f, 0.f);
     B[k] = make_float2(0.f, 0.f);
   }
-  mfma_f4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+  mfma_f4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f}, cre2 = cre, cim2 = cim;
+  // staging: TPR = ADJ_THREADS / CH threads per row, thread (row sk, part
+  // sp) loads 16 / TPR contiguous 16-byte words of the row's 2 KM slots
+  // (the first half of the parts cy, the second cx), so a chunk's rows
+  // arrive coalesced by all four waves; the next chunk's loads are issued
+  // before this chunk's matrix-core loop
+  constexpr int TPR = ADJ_THREADS / CH, FPT = 16 / TPR, SPT = 2 * FPT;   // float4 / slots per thread
+  static_assert(ADJ_THREADS % CH == 0 && 16 % TPR == 0 && KM == 16 && TPR >= 2, "row staging split");
+  const int sk = threadIdx.x / TPR, sp = threadIdx.x % TPR;
+  float4 pre[FPT];
+  float2 pv = make_float2(0.f, 0.f);
+  int pm = 0;
+  bool pok = false;
+  auto fetch = [&](int c) {
+    pok = sk < min(CH, it.z - c);
+    if (pok) {
+      const int r = g.tile_rows[c + sk];
+      const int cell = g.tile_cell[c + sk];
+      if (sp >= TPR / 2) pv = w(r);
+      const float4* src = reinterpret_cast<const float4*>(g.vals + (size_t)r * (2 * KM)) + FPT * sp;
+#pragma unroll
+      for (int j = 0; j < FPT; ++j) pre[j] = src[j];
+      pm = (cell % ncy - (g.Ky - 1) + 32) | ((cell / ncy - (g.Kx - 1) + 32) << 8);
+    }
+  };
+  fetch(it.y);
   for (int c0 = it.y; c0 < it.z; c0 += CH) {
     const int n = min(CH, it.z - c0);
-    const int i = threadIdx.x;
     __syncthreads();   // the previous chunk's reads are done (and the zeroing)
-    if (i < n) {
-      const int r = g.tile_rows[c0 + i];
-      const int cell = g.tile_cell[c0 + i];
-      const float2 v = w(r);
-      float2 cy[KM], cx[KM];
-      sep_load<KM>(g, r, cy, cx);
-      float2* qc = C + i * TM_STRIDE + TM_GAP;
-      float2* qb = B + i * TM_STRIDE + TM_GAP;
+    if (pok) {
+      const int s0 = (sp % (TPR / 2)) * SPT;   // first slot of this thread's part
+      if (sp < TPR / 2) {
+        float2* qc = C + sk * TM_STRIDE + TM_GAP + s0;
 #pragma unroll
-      for (int j = 0; j < KM; ++j) {
-        qc[j] = j < g.Ky ? make_float2(cy[j].x, -cy[j].y) : make_float2(0.f, 0.f);       // conj(cy)
-        qb[j] = j < g.Kx ? make_float2(cx[j].x * v.x + cx[j].y * v.y, cx[j].x * v.y - cx[j].y * v.x)
-                         : make_float2(0.f, 0.f);                                         // conj(cx) v
+        for (int j = 0; j < FPT; ++j) {   // conj(cy)
+          qc[2 * j] = s0 + 2 * j < g.Ky ? make_float2(pre[j].x, -pre[j].y) : make_float2(0.f, 0.f);
+          qc[2 * j + 1] = s0 + 2 * j + 1 < g.Ky ? make_float2(pre[j].z, -pre[j].w) : make_float2(0.f, 0.f);
+        }
+        if (sp == 0) meta[sk] = pm;
+      } else {
+        float2* qb = B + sk * TM_STRIDE + TM_GAP + s0;
+        const float2 v = pv;
+#pragma unroll
+        for (int j = 0; j < FPT; ++j) {   // conj(cx) v
+          qb[2 * j] = s0 + 2 * j < g.Kx ? make_float2(pre[j].x * v.x + pre[j].y * v.y, pre[j].x * v.y - pre[j].y * v.x)
+                                        : make_float2(0.f, 0.f);
+          qb[2 * j + 1] = s0 + 2 * j + 1 < g.Kx
+                              ? make_float2(pre[j].z * v.x + pre[j].w * v.y, pre[j].z * v.y - pre[j].w * v.x)
+                              : make_float2(0.f, 0.f);
+        }
       }
-      meta[i] = (cell % ncy - (g.Ky - 1) + 32) | ((cell / ncy - (g.Kx - 1) + 32) << 8);
     }
     __syncthreads();
     // this wave's rows (window meets its quadrant), listed as the slot
@@ -400,6 +434,7 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
     // rows missing from the last group of four read the leading zero gap
     if (lane < 3) mine[nh + lane] = 0;
     __syncthreads();
+    if (c0 + CH < it.z) fetch(c0 + CH);
     // rank-1 updates, two rows per instruction pair and two pairs per step
     // (their LDS reads issued together): A lane (i, k) holds re / im (k
     // even / odd) of C at its row's offset + i, B lane (j, k) the matching
@@ -410,10 +445,12 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
       const float2 c1 = C[(e1 & 0xffff) + li], b1 = B[(e1 >> 16) + li];
       cre = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c0.y : c0.x, odd ? -b0.y : b0.x, cre, 0, 0, 0);
       cim = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c0.y : c0.x, odd ? b0.x : b0.y, cim, 0, 0, 0);
-      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c1.y : c1.x, odd ? -b1.y : b1.x, cre, 0, 0, 0);
-      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c1.y : c1.x, odd ? b1.x : b1.y, cim, 0, 0, 0);
+      cre2 = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c1.y : c1.x, odd ? -b1.y : b1.x, cre2, 0, 0, 0);
+      cim2 = __builtin_amdgcn_mfma_f32_16x16x4f32(odd ? c1.y : c1.x, odd ? b1.x : b1.y, cim2, 0, 0, 0);
     }
   }
+  cre += cre2;   // the two pairs' chains (four independent accumulators in the loop)
+  cim += cim2;
   // lane (j, k) holds rows 4k .. 4k + 3 of column j of the quadrant
   const int lx = qx0 + li, ly0 = qy0 + 4 * (lane >> 4);
   if (it.w >= 0) {   // one of several items of this tile: partial sums
