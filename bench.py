@@ -376,7 +376,14 @@ def cube_main(args):
                for k in rep}
     del engines, eng
     torch.cuda.synchronize()
-    # ---- time to reference SNR: per sub-cube, plain (drop-in) and batched --
+    # HBM bytes per launch of the dominant kernel on sub-cube 0, from the
+    # copy-calibrated FETCH_SIZE / WRITE_SIZE passes of this workload
+    traffic = None
+    tpath = os.path.join(ROOT, 'profiles', 'fhs2048x64_pmc_traffic.json')
+    if os.path.exists(tpath) and dict(CUBE_2048x64, fov=args.fov) == cfg and args.seed == 0:
+        with open(tpath) as fh:
+            traffic = json.load(fh).get('traffic_bytes_per_launch', {}).get(dominant)
+    # ---- time to reference SNR: per sub-cube, plain (drop-in) and chosen --
     solve = None
     fx_path = os.path.join(ROOT, 'tests', 'golden', 'fhs2048x64_reference.npz')
     default_cfg = dict(CUBE_2048x64, fov=args.fov) == cfg and args.seed == 0
@@ -455,7 +462,9 @@ def cube_main(args):
                    'Mtot': int(sum(m.D.Mtot for (_, m, _, _) in subs)), 'nnz_G_total': nnz,
                    'setup_s': round(setup_s, 2)},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
-                     'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
+                     'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
+                     'traffic': None if traffic is None else int(traffic),
+                     'traffic_source': 'profiles/fhs2048x64_pmc_traffic.json (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)',
                      'avg_launch_ms': round(avg_ms, 5), 'algorithmic_bytes_per_launch': int(alg[dominant]),
                      'measured_on': 'sub-cube 0 (16 channels)'},
         'kernels': kernels,
