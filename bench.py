@@ -61,8 +61,8 @@ def parse():
     ap.add_argument('--solve-repeats', type=int, default=3,
                     help='timed solves per start after the full reference solve; the median time is reported')
     ap.add_argument('--copy-calibration', action='store_true',
-                    help='only run a plain 256 MiB device-to-device copy 3 times (the reference kernel for '
-                         'scaling the FETCH_SIZE / WRITE_SIZE counters) and exit')
+                    help='only run a plain 256 MiB device-to-device copy and a 256 MiB elementwise add, 3 times '
+                         'each (the reference kernels for scaling the FETCH_SIZE / WRITE_SIZE counters), and exit')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--gram', choices=('fp64', 'bf16x3', 'bf16x2'), default='bf16x3',
                     help='nuclear-norm Gram precision of the chosen form (throughput engine and chosen solves)')
