@@ -9,10 +9,13 @@
 // lf): out[k] = sum_j f[j] in[2k + 2 - lf + j], n_out = floor((n + lf - 1)/2).
 // a = lo_y lo_x, h = hi_y lo_x, v = lo_y hi_x, d = hi_y hi_x (y = rows).
 //
-// Kernels work on T x T output tiles: the input window (plus filter halo) is
-// staged in LDS, filtered along x then y (separable, 2 lf FMAs per output).
-// Analysis runs one workgroup per (tile, basis, channel) at every level
-// (level 1 re-reads the image window per basis from L2).  Synthesis mirrors the analysis,
+// Kernels work on output tiles (analysis 32 x 16, synthesis 32 x 32): the
+// input window (plus filter halo) is staged in LDS, filtered along x then y
+// (separable, 2 lf FMAs per output).  Analysis runs one workgroup per (tile,
+// basis, channel) at every level; its 32 x 16 tiles keep the LDS at 24 KB,
+// so five workgroups fit on a CU instead of three with 32 x 32 tiles (one
+// workgroup per tile running all the bases from one image window measured
+// slower: the bases then run one after the other).  Synthesis mirrors the analysis,
 // and its finest level sums all bases and the Dirac part in registers so the
 // image is written once.
 #include <algorithm>
@@ -67,7 +70,9 @@ struct AnOut {
   bool last;
 };
 
-constexpr int AN_PER = WT * WT / WTHREADS;   // outputs per thread
+// analysis tiles: AY x AX outputs (rows x columns)
+constexpr int AY = 32, AX = 16;
+constexpr int AN_PER = AY * AX / WTHREADS;   // outputs per thread
 
 // old values of the coefficients a tile accumulates into (+=), loaded at
 // kernel start so that their latency overlaps the window load
@@ -79,7 +84,7 @@ __device__ __forceinline__ void an_prefetch(int ky0, int kx0, const AnOut& out, 
 #pragma unroll
   for (int q = 0; q < AN_PER; ++q) {
     const int t = threadIdx.x + q * WTHREADS;
-    const int gy = ky0 + t % WT, gx = kx0 + t / WT;
+    const int gy = ky0 + t % AY, gx = kx0 + t / AY;
     pre.a[q] = pre.h[q] = pre.v[q] = pre.d[q] = 0.f;
     if (gy < out.ny && gx < out.nx) {
       const int64_t p = gy + (int64_t)out.ny * gx;
@@ -95,15 +100,15 @@ __device__ __forceinline__ void an_prefetch(int ky0, int kx0, const AnOut& out, 
 template <int LF>
 __device__ void an_tile_t(const float* win, int WH, const float* lo, const float* hi, int ky0, int kx0, float* Lx,
                           float* Hx, const AnOut& out, const AnPre& pre) {
-  constexpr int H = 2 * WT + LF - 2;   // rows of the window
+  constexpr int H = 2 * AY + LF - 2;   // rows of the window
   float rlo[LF], rhi[LF];
 #pragma unroll
   for (int j = 0; j < LF; ++j) {
     rlo[j] = lo[j];
     rhi[j] = hi[j];
   }
-  // x pass: Lx[kx][r], Hx[kx][r] for r < H, kx < WT
-  for (int t = threadIdx.x; t < H * WT; t += WTHREADS) {
+  // x pass: Lx[kx][r], Hx[kx][r] for r < H, kx < AX
+  for (int t = threadIdx.x; t < H * AX; t += WTHREADS) {
     const int r = t % H, kx = t / H;
     float sl = 0.f, sh = 0.f;
     const float* col = win + (size_t)(2 * kx) * WH + r;
@@ -121,7 +126,7 @@ __device__ void an_tile_t(const float* win, int WH, const float* lo, const float
 #pragma unroll
   for (int q = 0; q < AN_PER; ++q) {
     const int t = threadIdx.x + q * WTHREADS;
-    const int ky = t % WT, kx = t / WT;
+    const int ky = t % AY, kx = t / AY;
     const int gy = ky0 + ky, gx = kx0 + kx;
     if (gy < out.ny && gx < out.nx) {
       const float* lc = Lx + kx * H + 2 * ky;
@@ -163,8 +168,9 @@ __device__ void an_tile(const float* win, int WH, int lf, const float* lo, const
   }
 }
 
-constexpr int WIN_MAX = 2 * WT + WMAX_LF - 2;   // 78
-constexpr int WIN_PER_THREAD = (WIN_MAX * WIN_MAX + WTHREADS - 1) / WTHREADS;   // 24
+constexpr int WIN_ROWS = 2 * AY + WMAX_LF - 2;   // 78
+constexpr int WIN_COLS = 2 * AX + WMAX_LF - 2;   // 46
+constexpr int WIN_PER_THREAD = (WIN_ROWS * WIN_COLS + WTHREADS - 1) / WTHREADS;
 
 // load window [wy0, wy0+WH) x [wx0, wx0+WW) of a column-major ny x nx image;
 // every load of a thread is issued before the first LDS store, so the
@@ -193,19 +199,19 @@ __device__ void load_window(float* win, const float* img, int ny, int nx, int wy
 
 __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const float* __restrict__ x,
                                                     float* __restrict__ v1, float* __restrict__ scr) {
-  __shared__ float win[WIN_MAX * WIN_MAX];
-  __shared__ float Lx[WT * WIN_MAX];
-  __shared__ float Hx[WT * WIN_MAX];
+  __shared__ float win[WIN_ROWS * WIN_COLS];
+  __shared__ float Lx[AX * WIN_ROWS];
+  __shared__ float Hx[AX * WIN_ROWS];
   __shared__ float flo[WMAX_LF], fhi[WMAX_LF];
   const int b = blockIdx.z % G.nb, l = blockIdx.z / G.nb;
-  const int ky0 = blockIdx.x * WT, kx0 = blockIdx.y * WT;
+  const int ky0 = blockIdx.x * AY, kx0 = blockIdx.y * AX;
   if (ky0 >= G.ny[b][k] || kx0 >= G.nx[b][k]) return;
   const int lf = G.lf[b];
   if (threadIdx.x < lf) {
     flo[threadIdx.x] = G.lo[b][threadIdx.x];
     fhi[threadIdx.x] = G.hi[b][threadIdx.x];
   }
-  const int WH = 2 * WT + lf - 2;
+  const int WH = 2 * AY + lf - 2, WW = 2 * AX + lf - 2;   // window rows, columns
   float* sc = scr + (int64_t)l * G.scr_per_chan;
   AnOut out;
   out.v1 = v1 + (int64_t)l * G.s;
@@ -218,11 +224,11 @@ __global__ __launch_bounds__(WTHREADS) void k_an_lk(SaraGeom G, int k, const flo
   AnPre pre;
   an_prefetch(ky0, kx0, out, pre);
   if (k == 1)   // level 1 reads the image, scaled by 1/sqrt(P)
-    load_window(win, x + (int64_t)l * G.Ny * G.Nx, G.Ny, G.Nx, 2 * ky0 + 2 - lf, 2 * kx0 + 2 - lf, WH, WH,
+    load_window(win, x + (int64_t)l * G.Ny * G.Nx, G.Ny, G.Nx, 2 * ky0 + 2 - lf, 2 * kx0 + 2 - lf, WH, WW,
                 G.norm);
   else
     load_window(win, sc + G.scr[b][k - 1], G.ny[b][k - 1], G.nx[b][k - 1], 2 * ky0 + 2 - lf,
-                2 * kx0 + 2 - lf, WH, WH, 1.f);
+                2 * kx0 + 2 - lf, WH, WW, 1.f);
   an_tile(win, WH, lf, flo, fhi, ky0, kx0, Lx, Hx, out, pre);
 }
 
@@ -743,7 +749,7 @@ void sara_analysis_wavelets(fhs_ctx* ctx, const fhs_sara* S, const float* x, int
       ny = std::max(ny, G.ny[b][k]);
       nx = std::max(nx, G.nx[b][k]);
     }
-    k_an_lk<<<dim3(cdiv(ny, WT), cdiv(nx, WT), G.nb * L), WTHREADS, 0, ctx->stream>>>(G, k, x, v1, scr);
+    k_an_lk<<<dim3(cdiv(ny, AY), cdiv(nx, AX), G.nb * L), WTHREADS, 0, ctx->stream>>>(G, k, x, v1, scr);
     check_launch();
   }
 }
