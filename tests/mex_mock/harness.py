@@ -57,11 +57,35 @@ class Mex:
             ('mexmock_call', C.c_int, [C.c_void_p, C.c_int, C.POINTER(mx_p), C.c_int, C.POINTER(mx_p), C.c_char_p,
                                        C.c_int]),
             ('mexmock_run_at_exit', None, []),
+            ('mxGetString', C.c_int, [mx_p, C.c_char_p, sz]),
         ]:
             f = getattr(m, name)
             f.restype = res
             f.argtypes = args
         self._shims = {}
+        # MATLAB functions the shims call back: wfilters(name, 'r') -> the
+        # reconstruction low-pass filter (fhs.sara's Daubechies construction)
+        proto = C.CFUNCTYPE(C.c_int, C.c_int, C.POINTER(mx_p), C.c_int, C.POINTER(mx_p), C.c_char_p)
+
+        def matlab(nlhs, plhs, nrhs, prhs, name):
+            try:
+                if name.decode() != 'wfilters' or nrhs < 1:
+                    return 1
+                buf = C.create_string_buffer(64)
+                m.mxGetString(prhs[0], buf, 64)
+                wname = buf.value.decode()
+                if not wname.startswith('db'):
+                    return 1
+                from fhs.sara import daubechies_scaling_filter
+                f = daubechies_scaling_filter(int(wname[2:]))
+                plhs[0] = self.to_mx(f[None, :])
+                return 0
+            except Exception:
+                return 1
+        self._matlab = proto(matlab)      # kept alive with the harness
+        m.mexmock_set_matlab.argtypes = [proto]
+        m.mexmock_set_matlab.restype = None
+        m.mexmock_set_matlab(self._matlab)
 
     # ---- conversions ----------------------------------------------------
     def to_mx(self, obj):

@@ -68,6 +68,7 @@ void mxDestroyArray(mxArray* a);
 void mexErrMsgIdAndTxt(const char* id, const char* fmt, ...);
 int mexAtExit(void (*fn)(void));
 void mexLock(void);
+int mexCallMATLAB(int nlhs, mxArray* plhs[], int nrhs, mxArray* prhs[], const char* name);
 
 #ifdef __cplusplus
 }

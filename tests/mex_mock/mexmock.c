@@ -296,6 +296,19 @@ int mexAtExit(void (*fn)(void)) {
 
 void mexLock(void) {}
 
+/* mexCallMATLAB: answered by a function the test harness registers (it
+   plays the MATLAB functions a shim calls back, e.g. wfilters) */
+typedef int (*mexmock_matlab_fn)(int nlhs, mxArray** plhs, int nrhs, mxArray** prhs, const char* name);
+static mexmock_matlab_fn g_matlab = NULL;
+
+void mexmock_set_matlab(mexmock_matlab_fn f) { g_matlab = f; }
+
+int mexCallMATLAB(int nlhs, mxArray* plhs[], int nrhs, mxArray* prhs[], const char* name) {
+  if (!g_matlab || g_matlab(nlhs, plhs, nrhs, prhs, name) != 0)
+    mexErrMsgIdAndTxt("mexmock:call", "mexCallMATLAB(%s) is not available", name);
+  return 0;
+}
+
 typedef void (*mex_fn)(int, mxArray**, int, const mxArray**);
 
 /* Runs fn like MATLAB: returns 0, or 1 with the error text in err. */

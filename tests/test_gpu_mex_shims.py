@@ -163,3 +163,51 @@ def test_fhs_update_dual_lowrankness(mex):
     # the threshold is active on some singular values and not on others
     s = np.linalg.svd(Z, compute_uv=False)
     assert np.any(s > beta0 * weights0) and np.any(s < beta0 * weights0)
+
+
+def test_faceted_wavelet_shims(mex):
+    """sdwt2_setup, sdwt2_sara_faceting and isdwt2_sara_faceting (the
+    faceted-wavelet submodule's functions, absent from the reference tree)
+    used exactly as src/fhs/fhs_update_dual_sparsity.m:65-93 uses them: the
+    facet's extended image built from dims_overlap_ref / offsetL / offsetR,
+    its coefficients against the oracle's faceted transform, their count
+    against the solver's own formula on Ncoefs, and the facets'
+    contributions over dims_overlap adding up to the whole-image synthesis."""
+    from oracle.wavelets import SaraFacets
+    rng = np.random.default_rng(3)
+    Ny, Nx, L = 64, 48, 2
+    ys, xs = [0, 20, 41, 64], [0, 25, 48]
+    Qy, Qx = len(ys) - 1, len(xs) - 1
+    Q = Qy * Qx
+    I = np.array([[ys[q % Qy], xs[q // Qy]] for q in range(Q)], dtype=float)
+    dims = np.array([[ys[q % Qy + 1] - ys[q % Qy], xs[q // Qy + 1] - xs[q // Qy]] for q in range(Q)], dtype=float)
+    wavelet = ['db1', 'db2', 'db3', 'db4', 'db5', 'db6', 'db7', 'db8', 'self']
+    flen = np.array([2, 4, 6, 8, 10, 12, 14, 16, 0], dtype=float)
+    nlevel = 4
+    (I_ref, dims_ref, I_ov, dims_ov, status, offset, offL, offR, Ncoefs, temL,
+     temR) = mex.call('sdwt2_setup', 11, np.array([[Ny, Nx]], dtype=float), I, dims, float(nlevel), wavelet, flen)
+    ref = SaraFacets(Ny, Nx, ys, xs)
+    x = rng.random((Ny, Nx, L))
+    c = rng.standard_normal((ref.glob.s, L))
+    total = np.zeros((Ny, Nx, L))
+    for q in range(Q):
+        oy, ox = int(I_ref[q, 0]), int(I_ref[q, 1])
+        ny, nx = int(dims_ref[q, 0]), int(dims_ref[q, 1])
+        assert np.array_equal(np.asarray(dims_ov[q]).ravel(), [ny, nx])
+        p = np.prod(np.asarray(Ncoefs[q]), axis=1)
+        sz = 3 * p.sum() - 2 * p[nlevel::nlevel + 1].sum() - 2 * p[-1]    # fhs_initialize_dual_and_weights.m:80-81
+        for l in range(L):
+            x_facet = x[oy:oy + ny, ox:ox + nx, l]
+            x_curr = np.zeros((ny + int(offL[q, 0] + offR[q, 0]), nx + int(offL[q, 1] + offR[q, 1])))
+            x_curr[int(offL[q, 0]):int(offL[q, 0]) + ny, int(offL[q, 1]):int(offL[q, 1]) + nx] = x_facet
+            z = mex.call('sdwt2_sara_faceting', 1, x_curr, I[q], offset, float(status[q, 0]), float(nlevel), wavelet,
+                         Ncoefs[q])
+            zr = ref.analysis(q, ref.extended(q, x[:, :, l]))
+            assert len(np.ravel(z)) == sz == len(zr)
+            assert _rel(np.ravel(z), zr) < 1e-5, (q, l)
+            g = mex.call('isdwt2_sara_faceting', 1, c[ref.facets[q]['global'], l], I[q], dims[q], I_ov[q], dims_ov[q],
+                         Ncoefs[q], float(nlevel), wavelet, temL[q], temR[q])
+            assert g.shape == (ny, nx)
+            total[oy:oy + ny, ox:ox + nx, l] += g
+    for l in range(L):
+        assert _rel(total[:, :, l], ref.glob.psi(c[:, l])) < 1e-5, l

@@ -17,7 +17,9 @@ import harness  # noqa: E402
 # the device handles they take
 SHADOWED = {'apply_direct_operator', 'apply_adjoint_operator', 'apply_scaled_fourier_transform',
             'compute_sara_prior', 'nuclear_norm', 'proj_l2ball', 'update_dual_data_fidelity', 'update_epsilon',
-            'fhs_update_primal', 'fhs_update_dual_lowrankness'}
+            'fhs_update_primal', 'fhs_update_dual_lowrankness',
+            # the faceted-wavelet submodule's functions (absent from the reference tree)
+            'sdwt2_setup', 'sdwt2_sara_faceting', 'isdwt2_sara_faceting'}
 HELPERS = {'fhs_gmatrix', 'fhs_data_state', 'fhs_sara_dictionary'}
 
 
@@ -44,6 +46,8 @@ def test_errors_come_back_through_mexerrmsg(mex):
         mex.call('nuclear_norm', 1)
     with pytest.raises(harness.MexError, match='fhs:(ctx|args)'):
         mex.call('update_dual_data_fidelity', 7, np.zeros((2, 2)))
+    with pytest.raises(harness.MexError, match='fhs:(ctx|args)'):
+        mex.call('sdwt2_setup', 11, np.array([[8.0, 8.0]]))
 
 
 def test_update_epsilon_shim_matches_the_oracle(mex):
