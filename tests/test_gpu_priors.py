@@ -115,8 +115,12 @@ def test_global_lowrank_step(ctx, Ny, Nx, L):
         assert rel(cube_from_dev(g0), 0.33 * v0.reshape((Ny, Nx, L), order='F')) < TOL
 
 
+# the last two are the faceted configurations' geometries: 1024x1024 in 4x4
+# facets with 25% overlap (spectral sub-cube of 16 channels) and 2048x2048 in
+# 6x6 facets with 50% overlap
 @pytest.mark.parametrize('Ny,Nx,Qy,Qx,frac,L', [(64, 96, 2, 3, 0.25, 5), (120, 120, 3, 3, 0.5, 16),
-                                                (256, 256, 4, 4, 0.25, 16)])
+                                                (256, 256, 4, 4, 0.25, 16), (1024, 1024, 4, 4, 0.25, 16),
+                                                (2048, 2048, 6, 6, 0.5, 16)])
 def test_faceted_lowrank_step(ctx, Ny, Nx, Qy, Qx, frac, L):
     import fhs
     rng = np.random.default_rng(Qy * Qx + L)

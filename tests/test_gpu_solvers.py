@@ -102,6 +102,34 @@ def test_facet_hypersara_matches_oracle(ctx):
     assert abs(snr(prob.X0, x_gpu) - snr(prob.X0, x_ref)) < 0.1
 
 
+def test_facet_hypersara_iterations_at_1024_geometry(ctx):
+    """Faceted HyperSARA iterations at the 1024x1024 configuration's facet
+    geometry (4x4 facets, 25% overlap, uneven facet sizes) against the
+    oracle's: three channels, three iterations, all the steps (primal,
+    faceted low-rank, sparsity, preconditioned data step)."""
+    from fhs import solvers, synth
+    from oracle.facets import facet_geometry
+    from oracle.solvers import facet_hypersara as fhs_ref
+    N, L = 1024, 3
+    prob = synth.make_problem(N, N, L, 40000, 2, seed=23, n_sources=60)
+    meas = solvers.Measurement(ctx, prob, precond=True)
+    son, nu2 = meas.op_norm(False), meas.op_norm(True)
+    geom = facet_geometry(N, N, 4, 4, (0.25, 0.25))
+    K = 3
+    x_gpu, info = solvers.facet_hyper_sara(meas, geom, {'squared_operator_norm': son, 'nu2': nu2},
+                                           max_total_iter=K, graph=False)
+    op = oracle_problem(prob, meas)
+    pix = [geom['py'][q] * geom['px'][q] for q in range(16)]
+    sig, sig_bar = solvers.noise_levels_fhs(L, np.ones(L), son, pix)
+    pr = dict(solvers.DEFAULT_PARAM)
+    pr.update({'nu2': nu2, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar, 'alpha': 1.0})
+    x_ref, rinfo = fhs_ref(op, pr, geom, max_iter_total=K)
+    rel = np.linalg.norm(x_gpu - x_ref) / np.linalg.norm(x_ref)
+    print(f'FHS 1024x1024x{L}, 4x4 facets, {K} iterations: rel {rel:.2e}')
+    assert rel < 1e-4, rel
+    assert np.allclose(info['rel_val'][1:], rinfo['rel_val'][1:], rtol=1e-3)
+
+
 def test_concurrent_streams_match_sequential(ctx):
     """Priors forked onto the side stream give the same iterates as in order."""
     from fhs import solvers, synth
