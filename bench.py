@@ -197,10 +197,16 @@ def sara_against_oracle(args, solvers, meas, prob, son, nu2, n_iter=300):
                            [np.asarray(p, dtype=np.float64) for p in meas.pU_host], blocks, prob.epsilon,
                            g_scale=[1.0 / s for s in prob.sigma_noise])
         pr = dict(solvers.DEFAULT_PARAM)
-        pr.update({'nu2': nu2, 'gamma': float(np.sqrt(1.0 / son[0])), 'pdfb_max_iter': n_iter})
+        # the same loop as the GPU run: n_iter PDFB iterations, no reweighting
+        pr.update({k: prm[k] for k in ('pdfb_min_iter', 'pdfb_max_iter', 'reweighting_min_iter',
+                                       'reweighting_max_iter')})
+        pr.update({'nu2': nu2, 'gamma': float(np.sqrt(1.0 / son[0]))})
         tc = time.perf_counter()
         _, oinfo = sara_solver(op, pr, X0=prob.X0[:, :, 0], max_iter_total=n_iter)
         tcpu = time.perf_counter() - tc
+    if len(oinfo['snr']) != n_iter or full['iterations'] != n_iter or oinfo['reweights']:
+        raise RuntimeError(f"SARA reference runs differ in length: oracle {len(oinfo['snr'])} iterations, "
+                           f"{len(oinfo['reweights'])} reweights; GPU {full['iterations']}")
     ref = oinfo['snr'][-1]
     tols = (0.1, 0.5, 1.0, 2.0, 3.0)
     runs = [solvers._solve('sara', meas, prm, X0=prob.X0, stop_snr=ref - min(tols))[1]
