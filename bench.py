@@ -259,68 +259,98 @@ def gram_precision_errors(ctx, meas, facets, x, mode):
     return out
 
 
-def cpu_sample_cube(cfg, prob, meas, facets, threads):
-    """The FP64 oracle's cost of one PDFB iteration of the whole cube,
-    from a bounded sample: one channel's data step at full size (A, At and
-    the ellipsoid projection of its 4 blocks) with the first 1/16 of its
-    w-projected rows, the channel's wavelet analysis and synthesis, and one
-    facet's low-rank prox (SVD of the facet's 16 channels); scaled as
-    channels x (transforms + 16 x row-proportional part) + facets x prox."""
-    import torch
+_CPU_BARRIER = None
+
+
+def _cpu_pool_init(barrier):
+    global _CPU_BARRIER
+    _CPU_BARRIER = barrier
+
+
+def _cube_cpu_worker(task):
+    """One host worker of the cube's CPU sample: builds the sampled rows'
+    G, waits until every worker is ready, then times one channel's
+    transforms, rows and wavelets and one facet's prox (FP64 oracle, one
+    thread).  Returns (t_transforms, t_rows, t_wavelets, t_prox)."""
     import scipy.sparse as sp
+    from threadpoolctl import threadpool_limits
     from oracle.nufft import NufftOracle
     from oracle.prox import nuclear_prox_dual, solver_proj_elipse_fb
     from oracle.wavelets import SaraDictionary
+    d, seed = task
+    N, Kty, Ktx, a, cy, cx = d['N'], d['Kty'], d['Ktx'], d['a'], d['cy'], d['cx']
+    m, Ky, Kx = len(a), cy.shape[1], cx.shape[1]
+    ky = (a % Kty)[:, None] + np.arange(Ky)[None, :]
+    kx = (a // Kty)[:, None] + np.arange(Kx)[None, :]
+    cols = ((ky % Kty)[:, :, None] + Kty * (kx % Ktx)[:, None, :]).reshape(m, -1)
+    vals = (cy.astype(np.complex128)[:, :, None] * cx.astype(np.complex128)[:, None, :]).reshape(m, -1)
+    G = sp.csr_matrix((vals.ravel(), (np.repeat(np.arange(m), Ky * Kx), cols.ravel())), shape=(m, Kty * Ktx))
+    GH = G.conj().T.tocsr()
+    op = NufftOracle(np.zeros(1), np.zeros(1), N, N)
+    sara = SaraDictionary(N, N)
+    rng = np.random.default_rng(seed)
+    x = rng.random((N, N))
+    y = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    Z = rng.standard_normal((d['facet_pixels'], d['facet_channels']))
+    with threadpool_limits(1):
+        _CPU_BARRIER.wait(timeout=600)   # all workers time their phases together
+        t0 = time.perf_counter()
+        Fx = op.A(x)                                   # transforms: A and At
+        _ = np.real(op.At(Fx))
+        t_tr = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = G @ Fx                                     # rows: G, the ellipsoid projection, G'
+        z, _ = solver_proj_elipse_fb(np.zeros(m), r, y, np.ones(m), 0.5 * np.linalg.norm(r - y),
+                                     np.zeros(m, complex), 20, 1, 1e-8)
+        _ = GH @ (r - z)
+        t_rows = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        c = sara.psit(x)
+        _ = sara.psi(c)
+        t_wl = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nuclear_prox_dual(Z, np.ones(d['facet_channels']))
+        t_fac = time.perf_counter() - t0
+    return t_tr, t_rows, t_wl, t_fac
+
+
+def cpu_sample_cube(cfg, prob, meas, facets, pool, workers):
+    """The FP64 oracle's cost of one PDFB iteration of the whole cube on the
+    host's cores, from a bounded sample run by `workers` processes at once
+    (one core each): every worker times one channel's data step at full
+    size (A, At and the ellipsoid projection) on the first 1/16 of the
+    channel's w-projected rows, the channel's wavelet analysis and
+    synthesis, and one facet's low-rank prox (SVD of the facet's 16
+    channels).  A cube iteration takes ceil(channels / workers) rounds of
+    the slowest worker's channel (transforms + 16 x row-proportional part
+    + wavelets) plus ceil(facet proxes / workers) rounds of the prox."""
     N = cfg['N']
     C, L = cfg['subcubes'], cfg['channels']
     frac = 16
     m = len(prob.u[0]) // frac
     nf = meas.nufft
     a, cy, cx = nf.separable_wproj_gpu(meas.ctx, prob.u[0][:m], prob.v[0][:m], prob.w[0][:m], cfg['fov'])
-    Ky, Kx = cy.shape[1], cx.shape[1]
-    ky = (a % nf.Kty)[:, None] + np.arange(Ky)[None, :]
-    kx = (a // nf.Kty)[:, None] + np.arange(Kx)[None, :]
-    cols = ((ky % nf.Kty)[:, :, None] + nf.Kty * (kx % nf.Ktx)[:, None, :]).reshape(m, -1)
-    vals = (cy.astype(np.complex128)[:, :, None] * cx.astype(np.complex128)[:, None, :]).reshape(m, -1)
-    G = sp.csr_matrix((vals.ravel(), (np.repeat(np.arange(m), Ky * Kx), cols.ravel())), shape=(m, nf.No))
-    GH = G.conj().T.tocsr()
-    op = NufftOracle(np.zeros(1), np.zeros(1), N, N)
-    rng = np.random.default_rng(0)
-    x = rng.random((N, N))
-    y = rng.standard_normal(m) + 1j * rng.standard_normal(m)
-    pU = np.ones(m)
-    from threadpoolctl import threadpool_limits
-    lim = threadpool_limits(threads)
+    d = {'N': N, 'Kty': nf.Kty, 'Ktx': nf.Ktx, 'a': np.asarray(a), 'cy': np.asarray(cy), 'cx': np.asarray(cx),
+         'facet_pixels': max(facets['py']) * max(facets['px']), 'facet_channels': L // C}
     t0 = time.perf_counter()
-    Fx = op.A(x)                                   # transforms: A and At
-    _ = np.real(op.At(Fx))
-    t_tr = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    r = G @ Fx                                     # rows: G, the ellipsoid projection, G'
-    z, _ = solver_proj_elipse_fb(np.zeros(m), r, y, pU, 0.5 * np.linalg.norm(r - y), np.zeros(m, complex), 20, 1, 1e-8)
-    _ = GH @ (r - z)
-    t_rows = time.perf_counter() - t0
-    sara = SaraDictionary(N, N)
-    t0 = time.perf_counter()
-    c = sara.psit(x)
-    _ = sara.psi(c)
-    t_wl = time.perf_counter() - t0
-    P = facets['py'][0] * facets['px'][0]
-    Z = rng.standard_normal((max(facets['py']) * max(facets['px']), L // C))
-    t0 = time.perf_counter()
-    nuclear_prox_dual(Z, np.ones(L // C))
-    t_fac = time.perf_counter() - t0
-    lim.restore_original_limits()
+    res = np.array(pool.map(_cube_cpu_worker, [(d, s) for s in range(workers)], chunksize=1))
+    wall = time.perf_counter() - t0
+    t_tr, t_rows, t_wl, t_fac = (res[:, j] for j in range(4))
+    per_channel = t_tr + frac * t_rows + t_wl
     nq = len(facets['oy'])
-    t_iter = L * (t_tr + frac * t_rows + t_wl) + C * nq * t_fac
-    return {'value': round(1.0 / t_iter, 6), 'unit': 'iters/s', 'cores': threads, 'nproc': os.cpu_count(),
+    rounds = -(-L // workers)
+    t_iter = rounds * float(per_channel.max()) + (-(-C * nq // workers)) * float(t_fac.max())
+    Ky, Kx = cy.shape[1], cx.shape[1]
+    return {'value': round(1.0 / t_iter, 6), 'unit': 'iters/s', 'cores': workers, 'nproc': os.cpu_count(),
             'kind': 'port',
-            'sample': (f'FP64 numpy oracle on a bounded sample of the workload: one channel at 2048x2048 -- A and '
-                       f'At ({t_tr:.2f} s), G, the 20-step ellipsoid projection and G\' on 1/{frac} of its '
-                       f'{len(prob.u[0])} w-projected rows ({Ky}x{Kx} taps: {t_rows:.2f} s), the SARA analysis '
-                       f'and synthesis ({t_wl:.2f} s) -- and one facet\'s low-rank prox ({t_fac:.2f} s); one cube '
-                       f'iteration = {L} x (transforms + {frac} x rows + wavelets) + {C} x {nq} x prox '
-                       f'= {t_iter:.0f} s, one core')}
+            'sample': (f'FP64 numpy oracle, {workers} processes at once (one core each, {wall:.1f} s wall): each '
+                       f'times one channel at {N}x{N} -- A and At (median {np.median(t_tr):.2f} s), G, the 20-step '
+                       f'ellipsoid projection and G\' on 1/{frac} of its {len(prob.u[0])} w-projected rows '
+                       f'({Ky}x{Kx} taps: {np.median(t_rows):.2f} s), the SARA analysis and synthesis '
+                       f'({np.median(t_wl):.2f} s) -- and one facet\'s low-rank prox ({np.median(t_fac):.2f} s); '
+                       f'one cube iteration = {rounds} rounds of the slowest worker\'s channel (transforms + {frac} '
+                       f'x rows + wavelets, {per_channel.max():.1f} s) + {-(-C * nq // workers)} rounds of the prox '
+                       f'({C} x {nq} facets) = {t_iter:.0f} s on {workers} cores')}
 
 
 def cube_main(args):
@@ -333,6 +363,12 @@ def cube_main(args):
     if args.w_max is not None:
         cfg['w_max'] = args.w_max
     cfg['fov'] = args.fov
+    # the CPU baseline's worker processes are forked before the first HIP call
+    pool, workers = None, min(16, os.cpu_count() or 1)   # the box's CPU share for one GPU
+    if not args.no_cpu:
+        import multiprocessing as mp
+        fork = mp.get_context('fork')
+        pool = fork.Pool(workers, initializer=_cpu_pool_init, initargs=(fork.Barrier(workers),))
     ctx = fhs.default_context(0)
     ctx.set_autotune(not args.no_autotune)
     C = cfg['subcubes']
@@ -441,9 +477,10 @@ def cube_main(args):
     # ---- CPU baseline: bounded oracle sample --------------------------------
     cpu = None
     if not args.no_cpu:
-        threads = 1         # numpy's FFT and the wavelet filters run on one core; BLAS limited to match
         prob, meas, facets, _ = subs[0]
-        cpu = cpu_sample_cube(cfg, prob, meas, facets, threads)
+        cpu = cpu_sample_cube(cfg, prob, meas, facets, pool, workers)
+        pool.close()
+        pool.join()
         log('CPU baseline sample done')
     nnz = int(sum(g.nnz for (_, m, _, _) in subs for g in m.G))
     Ky = subs[0][1].G[0].Ky
