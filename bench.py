@@ -66,6 +66,9 @@ def parse():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--gram', choices=('fp64', 'bf16x3', 'bf16x2'), default='bf16x3',
                     help='nuclear-norm Gram precision of the chosen form (throughput engine and chosen solves)')
+    ap.add_argument('--rows', type=int, choices=(16, 32), default=16,
+                    help='storage precision of the G rows the data kernels read in the chosen form (throughput '
+                         'engine and chosen solves; tile-gridded separable G only, 32 for the DR form)')
     ap.add_argument('--no-autotune', action='store_true',
                     help='keep the default launch shapes (the tuned ones give bit-identical results)')
     ap.add_argument('--w-max', type=float, default=None,
@@ -128,16 +131,18 @@ def lowrank_bytes(lowrank, L, N):
 def algorithmic_bytes(meas, sara_s, lowrank=None):
     """Per-launch algorithmic HBM bytes of the kernels for the layouts in use
     (complex = 8 B, real FP32 = 4 B; every operand read or written once).
-    G is the separable min-max layout: Ky + Kx coefficients and an anchor
-    per row.  Its adjoint grids 32 x 32 tiles from per-tile row lists
-    (row index + local cell, 8 B per listed row): each row's coefficients,
-    v2 and pU counted once, each grid point written once."""
+    G is the separable min-max layout: Ky + Kx coefficients (8 B each, 4 B
+    when the data state reads FP16 rows) and an anchor per row.  Its adjoint
+    grids 32 x 32 tiles from per-tile row lists (row index + local cell, 8 B
+    per listed row): each row's coefficients, v2 and pU counted once, each
+    grid point written once."""
     L, No = meas.L, meas.nufft.No
     nf = meas.nufft
     M = [g.M for g in meas.G]
     N = meas.Ny * meas.Nx
     Ky, Kx = getattr(meas.G[0], 'Ky', nf.Ky), getattr(meas.G[0], 'Kx', nf.Kx)
-    row = 8 * (Ky + Kx) + 4
+    coef = 4 if getattr(getattr(meas, 'D', None), 'row_bits', 32) == 16 else 8
+    row = coef * (Ky + Kx) + 4
     fwd = sum(row * m + 8 * No + 24 * m for m in M)                      # coefs, Fx, gx_old r/w, r2
     adj = 0
     for a in meas.anchor:
@@ -257,6 +262,31 @@ def gram_precision_errors(ctx, meas, facets, x, mode):
     out['lowrank_g0_rel_err'] = float((res[mode][2] - res['fp64'][2]).norm() / res['fp64'][2].norm())
     out['reference'] = 'the FP64-Gram GPU path on the same cube (tested within 1e-6 of the FP64 oracle)'
     return out
+
+
+def row_precision_errors(meas, bits):
+    """Error of the data forward / adjoint with the G rows stored in `bits`
+    against the rows as created (FP32; those are tested within 1e-5 of the
+    FP64 oracle), on a random cube and random data: relative 2-norm errors."""
+    import torch
+    from fhs import solvers
+    lib, D, ctx = solvers.lib(), meas.D, meas.ctx
+    g = torch.Generator(device='cuda').manual_seed(1)
+    x = torch.rand((meas.L, meas.Nx, meas.Ny), dtype=torch.float32, device='cuda', generator=g)
+    y = torch.randn(D.Mtot, dtype=torch.complex64, device='cuda', generator=g)
+    prev = D.row_bits
+    res = {}
+    for b in (32, bits):
+        D.set_row_precision(b)
+        f = torch.empty(D.Mtot, dtype=torch.complex64, device='cuda')
+        a = torch.empty_like(x)
+        ctx.check(lib.fhs_data_forward(ctx.h, D.h, x.data_ptr(), None, f.data_ptr()), 'data_forward')
+        ctx.check(lib.fhs_data_adjoint(ctx.h, D.h, y.data_ptr(), None, None, a.data_ptr()), 'data_adjoint')
+        res[b] = (f.to(torch.complex128), a.double())
+    D.set_row_precision(prev)
+    rel = lambda u, v: float((u - v).norm() / v.norm())  # noqa: E731
+    return {'forward_rel_err': rel(res[bits][0], res[32][0]), 'adjoint_rel_err': rel(res[bits][1], res[32][1]),
+            'reference': 'the same data state with FP32 rows (tested within 1e-5 of the FP64 oracle)'}
 
 
 _CPU_BARRIER = None
@@ -387,7 +417,7 @@ def cube_main(args):
             holder['eng'] = eng
             return t < args.warmup
         solvers._solve('fhs', meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1,
-                       gram=args.gram)
+                       gram=args.gram, rows=args.rows)
         engines.append(holder['eng'])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -439,7 +469,8 @@ def cube_main(args):
             prm = dict(param, track_snr=True)
             row = {'ref_snr_db': round(ref, 4), 'reference_iterations': int(fx[f'iterations_{c}']),
                    'reference_reweights': [int(t) for t in fx[f'reweights_{c}']]}
-            for name, kw in (('plain', dict(form='dropin')), ('chosen', dict(form='engine', gram=args.gram))):
+            for name, kw in (('plain', dict(form='dropin')),
+                             ('chosen', dict(form='engine', gram=args.gram, rows=args.rows))):
                 eng_s, info = solvers._solve('fhs', meas, prm, X0=prob.X0, facets=facets, stop_snr=ref - min(tols),
                                              time_limit=args.solve_seconds, **kw)
                 reach = {}
@@ -447,8 +478,11 @@ def cube_main(args):
                     k = warmstart.iterations_to(info['snr'], ref - d)
                     reach[f'ref-{d}dB'] = None if k is None else {'iterations': k, 's': round(info['time'][k - 1], 4)}
                 row[name] = reach
-                if name == 'chosen' and c == 0 and args.gram != 'fp64':
-                    mixed_err = gram_precision_errors(ctx, meas, facets, eng_s.x, args.gram)
+                if name == 'chosen' and c == 0:
+                    mixed_err = gram_precision_errors(ctx, meas, facets, eng_s.x, args.gram) \
+                        if args.gram != 'fp64' else {}
+                    if args.rows != 32:
+                        mixed_err['g_rows'] = dict(row_precision_errors(meas, args.rows), bits=args.rows)
                 del eng_s
             per.append(row)
             log(f'sub-cube {c}: timed solves done')
@@ -469,7 +503,8 @@ def cube_main(args):
                            'tolerance when every sub-cube has, at the sum of their times',
                  'plain_form': 'drop-in entry points per iteration, eager, FP64 Gram, zero start',
                  'chosen_form': (f'PDFB engine (one hipGraph replay per iteration, priors on side streams, stopping '
-                                 f'rule in the library), {args.gram} Gram on the matrix cores, zero start'),
+                                 f'rule in the library), {args.gram} Gram on the matrix cores, FP{args.rows} G '
+                                 f'rows, zero start'),
                  'warm_start': ('not part of the chosen form: the learned start does not shorten the time to '
                                 'ref - 0.1 dB at 512x512x16, and the shipped network is trained on 512x512x16 cubes'),
                  'mixed_precision_errors': mixed_err,
@@ -495,7 +530,7 @@ def cube_main(args):
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': f'f32 (f64 norms, {args.gram} Gram, f64 eig)',
+        'dtype': f'f32 (f64 norms, {args.gram} Gram, f64 eig, fp{args.rows} G rows)',
         'data': 'synthetic',
         'config': {'workload': (f'Faceted HyperSARA {cfg["N"]}x{cfg["N"]}x{cfg["channels"]} as C = {C} spectral '
                                 f'sub-cubes of {cfg["channels"] // C} channels solved on one GPU (one step = one PDFB '
@@ -599,7 +634,10 @@ def main():
         holder['eng'] = eng
         return t < args.warmup
 
-    solvers._solve(kind, meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1, gram=args.gram)
+    # the chosen form's G rows (FP32 for SARA, which has no chosen form, and the DR form's untiled G)
+    rows_c = 32 if (kind == 'sara' or dr) else args.rows
+    solvers._solve(kind, meas, param, facets=facets, callback=grab, max_total_iter=args.warmup + 1, gram=args.gram,
+                   rows=rows_c)
     eng = holder['eng']
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -710,12 +748,15 @@ def main():
         if kind != 'sara':
             # chosen: the engine with the mixed-precision Gram (matrix cores)
             eng_c, chosen = solvers._solve(kind, meas, prm, X0=prob.X0, facets=facets,
-                                           time_limit=args.solve_seconds, gram=args.gram)
-            mixed_err = gram_precision_errors(ctx, meas, facets, eng_c.x, args.gram) if args.gram != 'fp64' else None
+                                           time_limit=args.solve_seconds, gram=args.gram, rows=rows_c)
+            mixed_err = gram_precision_errors(ctx, meas, facets, eng_c.x, args.gram) if args.gram != 'fp64' else {}
+            if rows_c != 32:
+                mixed_err['g_rows'] = dict(row_precision_errors(meas, rows_c), bits=rows_c)
             del eng_c
             log(f'chosen-form solve: {chosen["iterations"]} iterations, SNR {chosen["snr"][-1]:.4f} dB')
-            c_reach, c_all = reach(timed(gram=args.gram))
-            solve['chosen'] = {'form': f'PDFB engine as batched, with the {args.gram} Gram on the matrix cores',
+            c_reach, c_all = reach(timed(gram=args.gram, rows=rows_c))
+            solve['chosen'] = {'form': f'PDFB engine as batched, with the {args.gram} Gram on the matrix cores and '
+                                       f'FP{rows_c} G rows',
                                'iterations': chosen['iterations'], 'wall_s': round(chosen['wall'], 3),
                                'converged': chosen['converged'], 'reweights': chosen['reweights'],
                                'final_snr_db': round(chosen['snr'][-1], 4),
@@ -760,8 +801,11 @@ def main():
     if not args.no_cpu and args.config != 'sara256':
         from oracle.nufft import uniform_precond_weights
         from oracle.solvers import OracleProblem, pdfb
+        from threadpoolctl import threadpool_limits
         nproc = os.cpu_count() or 1
-        threads = min(16, nproc)     # the box's CPU share for one GPU
+        # one thread: the oracle loop's thread pool over channels and a
+        # threaded BLAS both measured slower (its wavelet code holds the GIL)
+        threads = 1
         blocks = []
         for l in range(prob.L):
             b = np.concatenate([[0], np.cumsum(prob.blocks[l])])
@@ -773,14 +817,16 @@ def main():
         pr = dict(solvers.DEFAULT_PARAM)
         pr.update({'nu2': nu2, 'gamma': sig, 'gamma0': sig_bar, 'sig': sig, 'sig_bar': sig_bar, 'alpha': 1.0})
         n_cpu = 10
-        tc = time.perf_counter()
-        pdfb('hs', op, pr, max_iter_total=n_cpu, threads=threads)
-        tcpu = time.perf_counter() - tc
-        log(f'CPU baseline: {n_cpu} oracle iterations on {threads} threads in {tcpu:.1f} s')
+        with threadpool_limits(threads):
+            tc = time.perf_counter()
+            pdfb('hs', op, pr, max_iter_total=n_cpu, threads=threads)
+            tcpu = time.perf_counter() - tc
+        log(f'CPU baseline: {n_cpu} oracle iterations on {threads} thread in {tcpu:.1f} s')
         cpu = {'value': round(n_cpu / tcpu, 5), 'unit': 'iters/s', 'cores': threads, 'nproc': nproc,
                'kind': 'port',
                'sample': f'{n_cpu} HyperSARA iterations of the same 512x512x16 problem by the FP64 numpy oracle '
-                         f'(channels on a pool of {threads} threads, BLAS threaded), {tcpu:.1f} s'}
+                         f'on one core (BLAS on one thread; a pool of threads over channels and a threaded BLAS '
+                         f'both ran slower), {tcpu:.1f} s'}
         if ref_fx is not None:
             from fhs import warmstart
             k = warmstart.iterations_to(list(ref_fx['snr']), float(ref_fx['snr'][-1]) - 0.1)
@@ -800,7 +846,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': f'f32 (f64 norms, {args.gram if kind != "sara" else "no"} Gram, f64 eig)',
+        'dtype': f'f32 (f64 norms, {args.gram if kind != "sara" else "no"} Gram, f64 eig, fp{rows_c} G rows)',
         'data': 'synthetic',
         'config': {'workload': (f'SARA {args.size}x{args.size}, {args.vis} visibilities, 2 blocks'
                                 if args.config == 'sara256' else

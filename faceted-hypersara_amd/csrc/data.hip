@@ -38,6 +38,9 @@ struct fhs_data {
   std::vector<int> blk_chan;           // [nblocks]
   bool square = false;                 // all G square (DR holographic form)
   fhs::GDev* gdev = nullptr;           // device [L]
+  std::vector<fhs::GDev> gd;           // host copy of gdev
+  std::vector<const float2*> rows32;   // each channel's G rows as created (FP32)
+  std::vector<void*> rows16;           // FP16 copies of the tiled separable rows (row precision 16)
   int64_t* d_row_off = nullptr;        // device [L+1]
   int64_t* d_blk_start = nullptr;      // device [nblocks+1]
   double* d_eps = nullptr;             // device [nblocks]
@@ -979,6 +982,9 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
                 "data_create: a block straddles channels");
     }
     D->gdev = dev_copy(gd.data(), L);
+    D->gd = gd;
+    for (int l = 0; l < L; ++l) D->rows32.push_back(gd[l].vals);
+    D->rows16.assign(L, nullptr);
     D->d_row_off = dev_copy(D->row_off.data(), L + 1);
     D->d_blk_start = dev_copy(D->blk_start.data(), nblocks + 1);
     FHS_HIP(hipMalloc(&D->d_eps, nblocks * sizeof(double)));
@@ -1004,9 +1010,56 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
   });
 }
 
+namespace fhs {
+namespace {
+// FP32 complex slots -> FP16 (re, im) pairs, round to nearest
+__global__ __launch_bounds__(256) void k_rows_to_half(const float2* __restrict__ in, __half2* __restrict__ out,
+                                                      int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = __float22half2_rn(in[i]);
+}
+}  // namespace
+}  // namespace fhs
+
+int fhs_data_set_row_precision(fhs_ctx* ctx, fhs_data* D, int bits) {
+  if (!ctx || !D) return FHS_ERR_ARG;
+  return guard(ctx, [&] {
+    FHS_CHECK(bits == 16 || bits == 32, "data_set_row_precision: bits must be 16 or 32");
+    FHS_HIP(hipSetDevice(D->device));
+    for (int l = 0; l < D->L; ++l) {
+      GDev& g = D->gd[l];
+      if (bits == 16) {
+        if (!g.tiled || g.layout != G_SEP)
+          throw Error(FHS_ERR_UNSUPPORTED, "data_set_row_precision: FP16 rows need a tile-gridded separable G");
+        if (!D->rows16[l]) {
+          const int64_t n = (int64_t)g.M * 2 * g.kmax;
+          FHS_HIP(hipMalloc(&D->rows16[l], n * sizeof(__half2)));
+          k_rows_to_half<<<cdiv(n, 256), 256, 0, ctx->stream>>>(D->rows32[l], (__half2*)D->rows16[l], n);
+          check_launch();
+        }
+        g.vals = (const float2*)D->rows16[l];
+        g.half = 1;
+      } else {
+        g.vals = D->rows32[l];
+        g.half = 0;
+      }
+    }
+    FHS_HIP(hipStreamSynchronize(ctx->stream));
+    FHS_HIP(hipMemcpy(D->gdev, D->gd.data(), D->L * sizeof(GDev), hipMemcpyHostToDevice));
+    if (bits == 32)
+      for (void*& p : D->rows16)
+        if (p) {
+          FHS_HIP(hipFree(p));
+          p = nullptr;
+        }
+  });
+}
+
 int fhs_data_destroy(fhs_data* D) {
   if (!D) return FHS_OK;
   (void)hipSetDevice(D->device);
+  for (void* p : D->rows16)
+    if (p) (void)hipFree(p);
   void* ps[] = {D->gdev, D->d_row_off, D->d_blk_start, D->d_eps, D->d_norm_res,
                 D->d_sigma22, D->Fx, D->g2, D->r2, D->fb_sync, D->fb_part, D->l2_part};
   for (void* p : ps)

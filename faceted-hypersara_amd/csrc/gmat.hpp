@@ -21,6 +21,8 @@
 // separable layout on grids of whole 32 x 32 tiles.
 #pragma once
 
+#include <hip/hip_fp16.h>
+
 #include "common.hpp"
 
 namespace fhs {
@@ -39,8 +41,9 @@ struct GDev {
   int ncols;
   const int* cols;      // ELL: [K][M]
   const int* anchor;    // WINDOW: [M]
-  const float2* vals;   // ELL / WINDOW: [K][M]; SEP: [M][2 kmax]
+  const float2* vals;   // ELL / WINDOW: [K][M]; SEP: [M][2 kmax] (__half2 slots if half)
   int kmax;             // SEP: complex slots per axis in a row (8 or 16)
+  int half;             // SEP on a tiled grid: rows stored as FP16 (re, im) pairs, 4 B a slot
   const int* colptr;    // [ncols + 1]
   const int* csc_rows;  // [nnz]
   const float2* csc_vals;
@@ -74,9 +77,33 @@ struct GDev {
   int nact;
 };
 
-// the 2 KM complex slots of separable row m (KM 16-byte loads); KM = g.kmax
+// one FP16 slot (re, im), as stored in a 32-bit word, in FP32
+__device__ __forceinline__ float2 h2f(unsigned u) {
+  __half2 h;
+  __builtin_memcpy(&h, &u, 4);
+  return __half22float2(h);
+}
+
+// the 2 KM complex slots of separable row m (KM 16-byte loads, KM / 2 for
+// FP16 rows); KM = g.kmax
 template <int KM>
 __device__ __forceinline__ void sep_load(const GDev& g, int m, float2* cy, float2* cx) {
+  if (g.half) {
+    const uint4* q = reinterpret_cast<const uint4*>(reinterpret_cast<const __half2*>(g.vals) + (size_t)m * (2 * KM));
+#pragma unroll
+    for (int i = 0; i < KM / 4; ++i) {
+      const uint4 a = q[i], b = q[KM / 4 + i];
+      cy[4 * i] = h2f(a.x);
+      cy[4 * i + 1] = h2f(a.y);
+      cy[4 * i + 2] = h2f(a.z);
+      cy[4 * i + 3] = h2f(a.w);
+      cx[4 * i] = h2f(b.x);
+      cx[4 * i + 1] = h2f(b.y);
+      cx[4 * i + 2] = h2f(b.z);
+      cx[4 * i + 3] = h2f(b.w);
+    }
+    return;
+  }
   const float4* p = reinterpret_cast<const float4*>(g.vals + (size_t)m * (2 * KM));
 #pragma unroll
   for (int i = 0; i < KM / 2; ++i) {
@@ -378,9 +405,18 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
       const int r = g.tile_rows[c + sk];
       const int cell = g.tile_cell[c + sk];
       if (sp >= TPR / 2) pv = w(r);
-      const float4* src = reinterpret_cast<const float4*>(g.vals + (size_t)r * (2 * KM)) + FPT * sp;
+      if (g.half) {   // the same 4 slots from one 16-byte word of FP16 pairs
+        static_assert(FPT == 2, "FP16 rows: 4 slots per thread");
+        const uint4 q = reinterpret_cast<const uint4*>(reinterpret_cast<const __half2*>(g.vals) +
+                                                       (size_t)r * (2 * KM))[sp];
+        const float2 a = h2f(q.x), b = h2f(q.y), c = h2f(q.z), d = h2f(q.w);
+        pre[0] = make_float4(a.x, a.y, b.x, b.y);
+        pre[1] = make_float4(c.x, c.y, d.x, d.y);
+      } else {
+        const float4* src = reinterpret_cast<const float4*>(g.vals + (size_t)r * (2 * KM)) + FPT * sp;
 #pragma unroll
-      for (int j = 0; j < FPT; ++j) pre[j] = src[j];
+        for (int j = 0; j < FPT; ++j) pre[j] = src[j];
+      }
       pm = (cell % ncy - (g.Ky - 1) + 32) | ((cell / ncy - (g.Kx - 1) + 32) << 8);
     }
   };

@@ -68,6 +68,7 @@ PROTOTYPES = {
     'fhs_data_create': [c_void_p, c_void_p, c_int, C.POINTER(c_void_p), c_int, c_i64_p, c_int_p,
                         C.POINTER(c_void_p)],
     'fhs_data_destroy': [c_void_p],
+    'fhs_data_set_row_precision': [c_void_p, c_void_p, c_int],
     'fhs_data_init_fx_old': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_data_fx_old_to_state': [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     'fhs_data_info': [c_void_p, c_int_p, c_int_p, c_int_p, c_i64_p, c_i64_p, c_int_p],

@@ -48,6 +48,14 @@ class DataOperator:
                                         bc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h)),
                   'data_create')
         self.h = h
+        self.row_bits = 32
+
+    def set_row_precision(self, bits):
+        """Storage precision of the G rows the data kernels read: 32 (as
+        created) or 16 (an FP16 copy, half the row bytes; tile-gridded
+        separable G only).  Call it between solves."""
+        self.ctx.check(lib().fhs_data_set_row_precision(self.ctx.h, self.h, int(bits)), 'data_set_row_precision')
+        self.row_bits = int(bits)
 
     def zeros(self):
         return torch.zeros(self.Mtot, dtype=torch.complex64, device=self.ctx.device)

@@ -345,7 +345,7 @@ def snr_db(X0, x):
 
 def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, graph=True,
            max_total_iter=None, time_limit=None, concurrent=True, weights_from_init=True, stop_snr=None,
-           form='engine', gram='fp64'):
+           form='engine', gram='fp64', rows=32):
     """Shared PDFB + reweighting driver.  kind: 'sara' | 'hs' | 'fhs'.
 
     With a nonzero ``x_init`` the reference derives the first weights from it
@@ -360,11 +360,17 @@ def _solve(kind, meas, param, x_init=None, X0=None, facets=None, callback=None, 
     as the sequence of reference-named entry points instead
     (``PdfbEngine.dropin_iterate``; SNR by a separate reduction).
     ``gram`` selects the nuclear-norm Gram precision
-    (``LowRankPrior.set_gram_precision``: 'fp64', 'bf16x3', 'bf16x2')."""
+    (``LowRankPrior.set_gram_precision``: 'fp64', 'bf16x3', 'bf16x2');
+    ``rows`` the storage precision of the G rows the data kernels read
+    (``DataOperator.set_row_precision``: 32 or 16).  It stays set on the
+    measurement after the call (the returned engine keeps using it); every
+    call sets it again."""
     prm = dict(DEFAULT_PARAM)
     prm.update(param or {})
     ctx = meas.ctx
     L, Ny, Nx = meas.L, meas.Ny, meas.Nx
+    if getattr(meas.D, 'row_bits', 32) != int(rows):
+        meas.D.set_row_precision(int(rows))
     sara = SaraDictionary(ctx, Ny, Nx)
     lowrank = None
     if kind == 'hs':

@@ -241,6 +241,16 @@ int fhs_data_create(fhs_ctx* ctx, const fhs_nufft* A, int L, const fhs_gmat* con
                     int nblocks, const int64_t* block_start, const int* block_channel,
                     fhs_data** out);
 int fhs_data_destroy(fhs_data* data);
+/* Storage precision of the interpolation rows the data state's kernels read
+ * (forward, adjoint, FB data step): bits = 32, the G rows as created, or
+ * 16, an FP16 copy of every channel's rows (4 B per complex slot: half the
+ * row bytes of the dominant gridding kernels; relative rounding 2^-11 per
+ * coefficient).  FP16 needs tile-gridded separable G on every channel
+ * (FHS_ERR_UNSUPPORTED otherwise).  No counterpart in the reference, whose
+ * G is double: a storage option of this library's mixed-precision form.
+ * The G objects themselves are unchanged; call it between solves, with no
+ * launch on the state in flight. */
+int fhs_data_set_row_precision(fhs_ctx* ctx, fhs_data* data, int bits);
 /* Sizes of a data state (any pointer may be NULL): image Ny x Nx, L
  * channels, No = Kty*Ktx grid points, Mtot stacked rows, data blocks. */
 int fhs_data_info(const fhs_data* data, int* Ny, int* Nx, int* L, int64_t* No, int64_t* Mtot, int* nblocks);
