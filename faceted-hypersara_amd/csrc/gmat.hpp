@@ -396,6 +396,7 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
   static_assert(ADJ_THREADS % CH == 0 && 16 % TPR == 0 && KM == 16 && TPR >= 2, "row staging split");
   const int sk = threadIdx.x / TPR, sp = threadIdx.x % TPR;
   float4 pre[FPT];
+  uint4 preh = make_uint4(0u, 0u, 0u, 0u);   // FP16 rows: the raw word, converted when staged
   float2 pv = make_float2(0.f, 0.f);
   int pm = 0;
   bool pok = false;
@@ -405,13 +406,8 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
       const int r = g.tile_rows[c + sk];
       const int cell = g.tile_cell[c + sk];
       if (sp >= TPR / 2) pv = w(r);
-      if (g.half) {   // the same 4 slots from one 16-byte word of FP16 pairs
-        static_assert(FPT == 2, "FP16 rows: 4 slots per thread");
-        const uint4 q = reinterpret_cast<const uint4*>(reinterpret_cast<const __half2*>(g.vals) +
-                                                       (size_t)r * (2 * KM))[sp];
-        const float2 a = h2f(q.x), b = h2f(q.y), c = h2f(q.z), d = h2f(q.w);
-        pre[0] = make_float4(a.x, a.y, b.x, b.y);
-        pre[1] = make_float4(c.x, c.y, d.x, d.y);
+      if (g.half) {   // the same 4 slots in one 16-byte word of FP16 pairs
+        preh = reinterpret_cast<const uint4*>(reinterpret_cast<const __half2*>(g.vals) + (size_t)r * (2 * KM))[sp];
       } else {
         const float4* src = reinterpret_cast<const float4*>(g.vals + (size_t)r * (2 * KM)) + FPT * sp;
 #pragma unroll
@@ -425,6 +421,12 @@ __device__ __forceinline__ void g_adjoint_tile_mfma(const GDev& g, int item, W&&
     const int n = min(CH, it.z - c0);
     __syncthreads();   // the previous chunk's reads are done (and the zeroing)
     if (pok) {
+      if (g.half) {
+        static_assert(FPT == 2, "FP16 rows: 4 slots per thread");
+        const float2 a = h2f(preh.x), b = h2f(preh.y), c = h2f(preh.z), d = h2f(preh.w);
+        pre[0] = make_float4(a.x, a.y, b.x, b.y);
+        pre[1] = make_float4(c.x, c.y, d.x, d.y);
+      }
       const int s0 = (sp % (TPR / 2)) * SPT;   // first slot of this thread's part
       if (sp < TPR / 2) {
         float2* qc = C + sk * TM_STRIDE + TM_GAP + s0;
