@@ -1026,11 +1026,13 @@ int fhs_data_set_row_precision(fhs_ctx* ctx, fhs_data* D, int bits) {
   return guard(ctx, [&] {
     FHS_CHECK(bits == 16 || bits == 32, "data_set_row_precision: bits must be 16 or 32");
     FHS_HIP(hipSetDevice(D->device));
+    if (bits == 16)   // every channel is checked before any is changed
+      for (int l = 0; l < D->L; ++l)
+        if (!D->gd[l].tiled || D->gd[l].layout != G_SEP)
+          throw Error(FHS_ERR_UNSUPPORTED, "data_set_row_precision: FP16 rows need a tile-gridded separable G");
     for (int l = 0; l < D->L; ++l) {
       GDev& g = D->gd[l];
       if (bits == 16) {
-        if (!g.tiled || g.layout != G_SEP)
-          throw Error(FHS_ERR_UNSUPPORTED, "data_set_row_precision: FP16 rows need a tile-gridded separable G");
         if (!D->rows16[l]) {
           const int64_t n = (int64_t)g.M * 2 * g.kmax;
           FHS_HIP(hipMalloc(&D->rows16[l], n * sizeof(__half2)));
