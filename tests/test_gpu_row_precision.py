@@ -132,3 +132,18 @@ def test_fp16_rows_refused(ctx):
     with pytest.raises(fhs.FhsError):
         D.set_row_precision(8)
     D.set_row_precision(32)
+    # a tiled separable channel next to a CSR one: refused, and the state unchanged
+    from fhs import solvers
+    a, cy, cx = nf.separable_gpu(ctx, u, v)
+    D2 = DataOperator(ctx, plan, [GMatrix.from_separable(ctx, a, cy, cx, nf.Kty, nf.Ktx),
+                                  GMatrix.from_scipy(ctx, op.G)], [[500], [500]])
+    x = torch.rand((2, N, N), dtype=torch.float32, device='cuda')
+
+    def fwd():
+        out = torch.empty(D2.Mtot, dtype=torch.complex64, device='cuda')
+        ctx.check(solvers.lib().fhs_data_forward(ctx.h, D2.h, x.data_ptr(), None, out.data_ptr()), 'data_forward')
+        return out.cpu().numpy()
+    before = fwd()
+    with pytest.raises(fhs.FhsError):
+        D2.set_row_precision(16)
+    assert D2.row_bits == 32 and np.array_equal(fwd(), before)
