@@ -52,6 +52,18 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
+def test_data_state_row_precision_argument(mex, problem):
+    """fhs_data_state's optional row_bits reaches fhs_data_set_row_precision:
+    32 is accepted; 16 is refused for these CSR G (FP16 rows need tile-gridded
+    separable G) with a MATLAB error, not a silent FP32 state."""
+    op = problem['ops'][0]
+    Gh = [mex.call('fhs_gmatrix', 1, o.G) for o in problem['ops']]
+    blocks = [np.array(BLOCKS, dtype=float) for _ in range(L)]
+    mex.call('fhs_data_state', 1, op.sy, op.sx, float(op.Kty), float(op.Ktx), Gh, blocks, 32.0)
+    with pytest.raises(harness.MexError, match='row_precision|FP16'):
+        mex.call('fhs_data_state', 1, op.sy, op.sx, float(op.Kty), float(op.Ktx), Gh, blocks, 16.0)
+
+
 def test_apply_scaled_fourier_transform(mex, problem):
     op = problem['ops'][0]
     x = problem['rng'].random((NY, NX, L))
