@@ -542,7 +542,10 @@ def cube_main(args):
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
                      'traffic': None if traffic is None else int(traffic),
-                     'traffic_source': 'profiles/fhs2048x64_pmc_traffic.json (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)',
+                     'traffic_source': ('profiles/fhs2048x64_pmc_traffic.json (FETCH_SIZE x 2 + WRITE_SIZE, '
+                                        'copy-calibrated; an upper bound for this gather kernel: its raw '
+                                        'FETCH_SIZE is 64 bytes per memory request, and the factor 2 fits the '
+                                        'streaming copy)'),
                      'avg_launch_ms': round(avg_ms, 5), 'algorithmic_bytes_per_launch': int(alg[dominant]),
                      'measured_on': 'sub-cube 0 (16 channels)'},
         'kernels': kernels,
