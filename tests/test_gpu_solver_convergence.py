@@ -127,13 +127,14 @@ def test_dropin_form_matches_engine(ctx, kind):
     assert np.allclose(np.array(a['rel_val']), np.array(b['rel_val']), rtol=1e-4, equal_nan=True)
 
 
-@pytest.mark.parametrize('gram', ['fp64', 'bf16x3'])
-def test_hs512x16_solve_matches_fp64_reference(ctx, gram):
+@pytest.mark.parametrize('gram,rows', [('fp64', 32), ('bf16x3', 32), ('bf16x3', 16)])
+def test_hs512x16_solve_matches_fp64_reference(ctx, gram, rows):
     """The benchmark workload (HyperSARA 512x512x16, 3 reweighting steps)
     solved on the GPU ends within 0.1 dB of the FP64 oracle's converged SNR
     on the same data, with the oracle's reweighting iterations (fixture of
     tests/golden/make_hs512_reference.py); also with the mixed-precision
-    (BF16 split, matrix-core) Gram of the chosen form."""
+    (BF16 split, matrix-core) Gram, and with it the FP16 G rows, of the
+    chosen form."""
     path = os.path.join(GOLDEN, 'hs512x16_reference.npz')
     if not os.path.exists(path):
         pytest.skip('no FP64 reference fixture for the 512x512x16 workload')
@@ -143,9 +144,9 @@ def test_hs512x16_solve_matches_fp64_reference(ctx, gram):
     synth.add_host_data(prob)
     meas = solvers.Measurement(ctx, prob, precond=True)
     p = {'squared_operator_norm': fx['son'], 'nu2': fx['nu2'], 'track_snr': True, 'reweighting_max_iter': 3}
-    _, info = solvers.hyper_sara(meas, p, X0=prob.X0, gram=gram)
+    _, info = solvers.hyper_sara(meas, p, X0=prob.X0, gram=gram, rows=rows)
     ref_rw = [int(t) for t in fx['reweights']]
-    print(f'GPU ({gram} Gram) {info["iterations"]} iterations, reweights {info["reweights"]}, SNR {info["snr"][-1]:.4f}; '
+    print(f'GPU ({gram} Gram, FP{rows} rows) {info["iterations"]} iterations, reweights {info["reweights"]}, SNR {info["snr"][-1]:.4f}; '
           f'oracle {int(fx["iterations"])}, {ref_rw}, {fx["snr"][-1]:.4f}', flush=True)
     assert info['converged'] and bool(fx['converged'])
     assert abs(info['snr'][-1] - float(fx['snr'][-1])) <= 0.1
